@@ -1,0 +1,236 @@
+/*
+ * opymize_amd C-ABI: hand-written HIP kernels for the PDHG hot path of
+ * Opymize (PDHG(G, F, A).solve on the GPU), built for MI355X (gfx950).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: 0 = ok, 1 = invalid argument,
+ *    2 = HIP launch/runtime error.  opy_last_error() returns a thread-local
+ *    message for the last failure.
+ *  - Array arguments are device pointers (flat, C order).  "_f32"/"_f64"
+ *    select the element type; FP64 accumulators are always double.
+ *  - `stream` is a hipStream_t passed as void*; every entry point only
+ *    enqueues work on it (no host synchronisation), so sequences of calls can
+ *    be captured into a hipGraph.
+ *  - `add` = 1 accumulates into the output (the reference's add=True),
+ *    0 overwrites it (the reference's fill(0) followed by +=).
+ *  - Outputs may alias inputs only where a function says so.
+ *
+ * The reference's device boundary is Operator.__call__ / LinOp.__call__
+ * dispatching to _call_gpu (opymize/__init__.py:64-120), which launches
+ * PyCUDA kernels compiled from opymize/{linear,operators}/*.cu and the
+ * string kernels of opymize/solvers/pdhg.py:61-79.  Each declaration below
+ * names the reference code it replaces.
+ */
+#ifndef OPYMIZE_AMD_H
+#define OPYMIZE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPY_ABI_VERSION 1
+#define OPY_MAX_DIMS 6
+#define OPY_RED_BLOCKS 1024   /* reductions need work buffers of 3*OPY_RED_BLOCKS doubles */
+#define OPY_QUADEPI_MAXM 16
+
+#define OPY_SCHEME_FORWARD 0
+#define OPY_SCHEME_CENTERED 1
+
+/* Cartesian grid of a finite-difference operator (reference diff.py:44-65). */
+typedef struct {
+  int32_t ndims;
+  int32_t nchannels;
+  int64_t dims[OPY_MAX_DIMS];
+  double h[OPY_MAX_DIMS];
+} opy_grid_t;
+
+/* Step size: per-element array `arr` (element type of the call), else the
+ * double at device address `dev`, else `host`. */
+typedef struct {
+  const void* arr;
+  const double* dev;
+  double host;
+} opy_step_t;
+
+/* Device-resident adaptive step state (reference pdhg.py:212-214, :155-183). */
+typedef struct {
+  double tau, sigma, alpha;
+  double resp, resd, epsp, epsd;
+  double sums[4]; /* squared sums: res_p, gnorm_p, res_d, gnorm_d */
+  int32_t nchanges;
+  int32_t pad;
+} opy_adapt_state_t;
+
+typedef struct {
+  double eta, Delta;
+  double sqrt_n, sqrt_m;
+  double eps_absp, eps_relp, eps_absd, eps_reld;
+} opy_adapt_params_t;
+
+/* Functional evaluation kinds for the FP64 primal-dual gap. */
+enum {
+  OPY_FN_SSD = 1,            /* SSD value                   (reference functionals/quad.py:33-44)      */
+  OPY_FN_L1DATA = 2,         /* sum lbd |x-f| (TV-L1 data term, new)                                 */
+  OPY_FN_L1DATA_CONJ = 3,    /* its conjugate: sum z f, infeas max(|z|-lbd)                          */
+  OPY_FN_NORMS = 4,          /* L1Norms value / L1NormsConj infeas (reference l1norms.py:14-70)     */
+  OPY_FN_DOT = 5,            /* AffineFct value             (reference functionals/affine.py:17-33) */
+  OPY_FN_INDICATOR = 6,      /* Indicator / Constrain infeas (reference indicators.py:17-46)        */
+  OPY_FN_MASKED_AFFINE = 7,  /* MaskedAffineFct             (reference functionals/affine.py:60-72) */
+  OPY_FN_POS = 8,            /* PositivityFct infeas        (reference indicators.py:57-63)         */
+  OPY_FN_NEG = 9,            /* NegativityFct infeas        (reference indicators.py:77-83)         */
+  OPY_FN_QUADEPI_IND = 10,   /* QuadEpiInd infeas           (reference epigraph.py:187-197)         */
+  OPY_FN_QUADEPI_SUPP = 11,  /* QuadEpiSupp value/infeas    (reference epigraph.py:154-166)         */
+  OPY_FN_HUBER = 12,         /* HuberPerspective            (reference epigraph.py:224-236)         */
+  OPY_FN_TRUNCQUADEPI = 13   /* TruncQuadEpiInd (2 infeas)  (reference epigraph.py:256-270)         */
+};
+
+typedef struct {
+  const void* f;         /* data term (element type of the call), per point x width */
+  const double* vol;     /* per-channel weights (SSD volume / L1 lambda), width entries */
+  const uint8_t* mask;   /* per-point mask or NULL */
+  const void* c;         /* per-entry constant (element type of the call) or NULL */
+  double cs;             /* scalar constant when c is NULL */
+  double lbd;            /* ball radius / Huber lambda */
+  int32_t m1, m2;        /* per-point matrix shape for OPY_FN_NORMS */
+  int32_t matrixnorm;    /* 'F' frobenius, 'N' nuclear (value) + spectral (infeas) */
+  int32_t pad;
+  double qa;             /* quadratic coefficient a (QuadEpi) or alph (Huber / TruncQuadEpi) */
+  const double* qb;      /* linear coefficients b (N x M) or NULL */
+  const double* qc;      /* constants c (N) or NULL */
+} opy_fn_params_t;
+
+/* ---- library ---------------------------------------------------------- */
+const char* opy_last_error(void);
+int opy_abi_version(void);
+int opy_device_sync(void);
+
+/* ---- finite differences (reference opymize/linear/diff.py, diff.cu) ----- */
+/* y (N,D,C) (+)= grad x with x (N,C); GradientOp._call_gpu, diff.py:141-144, diff.cu:37-75 */
+int opy_grad_f32(const float* x, float* y, const opy_grid_t* g, const double* w, int scheme, int add, void* stream);
+int opy_grad_f64(const double* x, double* y, const opy_grid_t* g, const double* w, int scheme, int add, void* stream);
+/* x (N,C) (+)= -div p with p (N,D,C); DivergenceOp._call_gpu, diff.py:180-183, diff.cu:77-127 */
+int opy_div_f32(const float* p, float* x, const opy_grid_t* g, const double* w, int scheme, int add, void* stream);
+int opy_div_f64(const double* p, double* x, const opy_grid_t* g, const double* w, int scheme, int add, void* stream);
+/* row-wise l^p norms of grad (adjoint=0) or div (adjoint=1); LinOp.rowwise_lp, __init__.py:122-142 */
+int opy_diff_rowwise_f32(float* out, const opy_grid_t* g, const double* w, int scheme, int adjoint, double p,
+                         int add, void* stream);
+int opy_diff_rowwise_f64(double* out, const opy_grid_t* g, const double* w, int scheme, int adjoint, double p,
+                         int add, void* stream);
+/* y (+)= Laplacian x, bc in {'N','C','S'}; LaplacianOp._call_gpu, diff.py:246-249, diff.cu:131-243 */
+int opy_laplacian_f32(const float* x, float* y, const opy_grid_t* g, int bc, int adj, int add, void* stream);
+int opy_laplacian_f64(const double* x, double* y, const opy_grid_t* g, int bc, int adj, int add, void* stream);
+int opy_laplacian_rowwise_f32(float* y, const opy_grid_t* g, int bc, int adj, double p, int add, void* stream);
+int opy_laplacian_rowwise_f64(double* y, const opy_grid_t* g, int bc, int adj, double p, int add, void* stream);
+
+/* ---- dense / sparse products (reference opymize/linear/einsum.py, einsum.cu) ---- */
+/* MatrixMult: y[j,i] (+)= sum_k A(j,k) x[k,i]; einsum.py:17-56, einsum.cu:3-28 */
+int opy_matmul_f32(const float* x, float* y, const float* A, int64_t N, int J, int K, int trans, int add, void* stream);
+int opy_matmul_f64(const double* x, double* y, const double* A, int64_t N, int J, int K, int trans, int add, void* stream);
+/* MatrixMultR: y[i,j] (+)= sum_k x[i,k] A(k,j); einsum.py:58-97, einsum.cu:32-57 */
+int opy_matmul_r_f32(const float* x, float* y, const float* A, int64_t N, int J, int K, int trans, int add, void* stream);
+int opy_matmul_r_f64(const double* x, double* y, const double* A, int64_t N, int J, int K, int trans, int add, void* stream);
+/* DiagMatrixMultR: y[i,k] (+)= x[i,k] A[k]; einsum.py:104-128 */
+int opy_diag_matmul_r_f32(const float* x, float* y, int64_t N, int K, const float* A, int add, void* stream);
+int opy_diag_matmul_r_f64(const double* x, double* y, int64_t N, int K, const double* A, int add, void* stream);
+/* CSR product for LinOps that only have a matrix (LinOp._call_cpu spmat.dot, __init__.py:95-108) */
+int opy_csr_spmv_f32(int64_t nrows, const int64_t* indptr, const int64_t* indices, const float* data, const float* x,
+                     float* y, int add, void* stream);
+int opy_csr_spmv_f64(int64_t nrows, const int64_t* indptr, const int64_t* indices, const double* data,
+                     const double* x, double* y, int add, void* stream);
+int opy_csr_rowwise_f32(int64_t nrows, const int64_t* indptr, const float* data, double p, float* y, int add,
+                        void* stream);
+int opy_csr_rowwise_f64(int64_t nrows, const int64_t* indptr, const double* data, double p, double* y, int add,
+                        void* stream);
+
+/* ---- trivial operators (reference opymize/linear/scale.py) ------------- */
+int opy_fill_f32(float* y, int64_t n, double value, void* stream);
+int opy_fill_f64(double* y, int64_t n, double value, void* stream);
+int opy_copy_f32(const float* x, float* y, int64_t n, void* stream);
+int opy_copy_f64(const double* x, double* y, int64_t n, void* stream);
+/* y[i] (+)= v[(i/div) % nv]: broadcast of per-row constants (rowwise norms of dense products) */
+int opy_tile_f32(float* y, int64_t n, const float* v, int64_t nv, int64_t div, int add, void* stream);
+int opy_tile_f64(double* y, int64_t n, const double* v, int64_t nv, int64_t div, int add, void* stream);
+/* y = |x| (normest start vector, reference linear/__init__.py:23) */
+int opy_abs_f32(const float* x, float* y, int64_t n, void* stream);
+int opy_abs_f64(const double* x, double* y, int64_t n, void* stream);
+/* x = 1/x where x > thr (diagonal preconditioners, reference pdhg.py:197-200) */
+int opy_recip_gt_f32(float* x, int64_t n, double thr, void* stream);
+int opy_recip_gt_f64(double* x, int64_t n, double thr, void* stream);
+/* ScaleOp / IdentityOp: y (+)= fact*x, fact array or scalar; scale.py:34-79 */
+int opy_scale_f32(const float* x, float* y, int64_t n, const float* fact, double fact_s, int add, void* stream);
+int opy_scale_f64(const double* x, double* y, int64_t n, const double* fact, double fact_s, int add, void* stream);
+/* NihilOp: y = keep ? x : 0; scale.py:96-115 */
+int opy_nihil_f32(const float* x, float* y, int64_t n, const uint8_t* keep, int add, void* stream);
+int opy_nihil_f64(const double* x, double* y, int64_t n, const uint8_t* keep, int add, void* stream);
+
+/* ---- prox maps (reference opymize/operators) ---------------------------- */
+/* ShiftScaleOp: y (+)= a*(x + b*shift), each of a, b, shift array or scalar; affine.py:85-146 */
+int opy_shiftscale_f32(const float* x, float* y, int64_t n, const float* shift, double shift_s, const float* a,
+                       double a_s, const float* b, double b_s, int add, void* stream);
+int opy_shiftscale_f64(const double* x, double* y, int64_t n, const double* shift, double shift_s, const double* a,
+                       double a_s, const double* b, double b_s, int add, void* stream);
+/* SSD.prox(tau): y = (x + t vol f)/(1 + t vol) on masked points; quad.py:45-55 (+ affine.py:120-123).
+ * The step comes from the device so adaptive steps need no host round trip. */
+int opy_ssd_prox_f32(const float* x, float* y, int64_t N, int C, const float* f, const double* vol,
+                     const uint8_t* mask, const opy_step_t* step, int add, void* stream);
+int opy_ssd_prox_f64(const double* x, double* y, int64_t N, int C, const double* f, const double* vol,
+                     const uint8_t* mask, const opy_step_t* step, int add, void* stream);
+/* TV-L1 data term prox (conj=0: soft threshold about f) or its conjugate's (conj=1: clip(x - t f)).
+ * No reference counterpart: the reference has no L1 data functional. */
+int opy_l1data_prox_f32(const float* x, float* y, int64_t N, int C, const float* f, const double* lbd, int conj,
+                        const opy_step_t* step, int add, void* stream);
+int opy_l1data_prox_f64(const double* x, double* y, int64_t N, int C, const double* f, const double* lbd, int conj,
+                        const opy_step_t* step, int add, void* stream);
+/* PosProj / NegProj: proj.py:19-85 */
+int opy_signproj_f32(const float* x, float* y, int64_t n, int positive, int add, void* stream);
+int opy_signproj_f64(const double* x, double* y, int64_t n, int positive, int add, void* stream);
+/* ConstOp: affine.py:15-46 */
+int opy_const_f32(float* y, int64_t n, const float* c, int add, void* stream);
+int opy_const_f64(double* y, int64_t n, const double* c, int add, void* stream);
+/* ConstrainOp: y = mask[i/M] ? c : x; affine.py:48-83 */
+int opy_constrain_f32(const float* x, float* y, int64_t n, int64_t M, const float* c, const uint8_t* mask, int add,
+                      void* stream);
+int opy_constrain_f64(const double* x, double* y, int64_t n, int64_t M, const double* c, const uint8_t* mask,
+                      int add, void* stream);
+/* L1NormsProj (in place, per-point M1 x M2 blocks stored column-major): proj.py:170-226, proj.cu:3-107 */
+int opy_l1norms_proj_f32(float* x, int64_t N, int M1, int M2, double lbd, int spectral, void* stream);
+int opy_l1norms_proj_f64(double* x, int64_t N, int M1, int M2, double lbd, int spectral, void* stream);
+/* QuadEpiProj (in place; shift may be NULL; lbd = +inf for no truncation): proj.py:278-374, proj.cu:110-219 */
+int opy_quadepi_proj_f32(float* x, int64_t N, int M, double alph, const float* shift, double lbd, void* stream);
+int opy_quadepi_proj_f64(double* x, int64_t N, int M, double alph, const double* shift, double lbd, void* stream);
+
+/* ---- PDHG step kernels (reference opymize/solvers/pdhg.py) ------------- */
+/* take_step: zn = z + dir*step*g; pdhg.py:62-65 */
+int opy_pdhg_take_step_f32(float* zn, const float* z, int64_t n, int dir, const opy_step_t* step, const float* g,
+                           void* stream);
+int opy_pdhg_take_step_f64(double* zn, const double* z, int64_t n, int dir, const opy_step_t* step, const double* g,
+                           void* stream);
+/* overrelax + dual take_step fused: yn = y + step*((1+theta) axn - theta ax); pdhg.py:66-69, :148-151 */
+int opy_pdhg_dual_step_f32(float* yn, const float* y, int64_t n, double theta, const opy_step_t* step,
+                           const float* axn, const float* ax, void* stream);
+int opy_pdhg_dual_step_f64(double* yn, const double* y, int64_t n, double theta, const opy_step_t* step,
+                           const double* axn, const double* ax, void* stream);
+/* residual: out2 = [sum((z-zn)/a - b g + c1 gn)^2, sum((z-zn)/a - b g + c2 gn)^2] in FP64;
+ * the two reductions of pdhg.py:74-78 per side, in one pass (work: 2*OPY_RED_BLOCKS doubles) */
+int opy_pdhg_residual2_f32(const opy_step_t* a, const float* z, const float* zn, double b, const float* g, double c1,
+                           double c2, const float* gn, int64_t n, double* work, double* out2, void* stream);
+int opy_pdhg_residual2_f64(const opy_step_t* a, const double* z, const double* zn, double b, const double* g,
+                           double c1, double c2, const double* gn, int64_t n, double* work, double* out2,
+                           void* stream);
+/* adaptive step rule on the device; pdhg.py:166-183 */
+int opy_pdhg_adapt(opy_adapt_state_t* state, const opy_adapt_params_t* params, void* stream);
+
+/* ---- FP64 functional evaluation for the gap (reference pdhg.py:23-44) --- */
+/* out3 = [sum of values, max infeasibility, second max infeasibility]; accumulate=1 adds/maxes into out3 */
+int opy_fn_eval_f32(int kind, const float* x, int64_t npts, int width, const opy_fn_params_t* prm, double* work,
+                    double* out3, int accumulate, void* stream);
+int opy_fn_eval_f64(int kind, const double* x, int64_t npts, int width, const opy_fn_params_t* prm, double* work,
+                    double* out3, int accumulate, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPYMIZE_AMD_H */

@@ -1,0 +1,96 @@
+"""Pins of the oracle PDHG (CPU only)."""
+import numpy as np
+import pytest
+
+from oracle import pdhg, tv1d
+
+
+def test_known_answer_three_iterations():
+    # 1x2 image, f = (0, 1), SSD volume 1, TV weight 1, constant steps with
+    # bound 1/4 -> sigma = tau = 1/2.  Worked by hand from the PDHG update
+    # x+ = prox(x - tau A^T y), y+ = proj(y + sigma (2 A x+ - A x)):
+    # it1: x = (0, 1/3), y = 1/3; it2: x = (1/9, 4/9), y = 1/2; it3: x = (13/54, 25/54), y = 5/9
+    f = np.array([[0.0], [1.0]])
+    prob = pdhg.tv_problem(f, (1, 2), 1.0)
+    expect = {1: ([0, 1 / 3], 1 / 3), 2: ([1 / 9, 4 / 9], 1 / 2), 3: ([13 / 54, 25 / 54], 5 / 9)}
+    for k, (xe, ye) in expect.items():
+        x, y, info, _ = pdhg.solve(prob, steps="const", step_bound=0.25, term_maxiter=k, granularity=10 ** 9)
+        np.testing.assert_allclose(x, xe, atol=1e-15)
+        np.testing.assert_allclose(y, [0, ye, 0, 0], atol=1e-15)
+
+
+def _rects(H, W, seed):
+    rng = np.random.default_rng(seed)
+    img = np.full((H, W), 0.5)
+    for _ in range(5):
+        y0, y1 = sorted(rng.integers(0, H, 2))
+        x0, x1 = sorted(rng.integers(0, W, 2))
+        img[y0:y1 + 1, x0:x1 + 1] = rng.random()
+    return img + 0.1 * rng.standard_normal((H, W))
+
+
+@pytest.mark.parametrize("steps", ["const", "adaptive", "precond"])
+def test_1d_tv_converges_to_exact_solution(steps):
+    W, lam = 40, 4.0
+    f = _rects(1, W, 0).reshape(-1, 1)
+    prob = pdhg.tv_problem(f, (1, W), lam)
+    x, y, info, _ = pdhg.solve(prob, steps=steps, step_bound=0.124, term_maxiter=20000, granularity=100,
+                               term_pd_gap=1e-12)
+    xs, _ = tv1d.tv1d_denoise(f.ravel(), 1.0 / lam)
+    # strong convexity of G: 0.5*lam*|x - x*|^2 <= gap
+    assert info["gap"] >= -1e-10
+    assert np.linalg.norm(x - xs) <= np.sqrt(2 * max(info["gap"], 0) / lam) + 1e-6
+
+
+def test_weak_duality_and_fixed_point_2d():
+    H, W, lam = 12, 14, 6.0
+    f = _rects(H, W, 1).reshape(-1, 1)
+    prob = pdhg.tv_problem(f, (H, W), lam)
+    gaps = []
+    for k in (10, 100, 400):
+        x, y, info, _ = pdhg.solve(prob, steps="const", step_bound=0.124, term_maxiter=k, granularity=k)
+        assert info["gap"] >= -1e-12 and info["infeasd"] <= 1e-12
+        gaps.append(info["gap"])
+    assert gaps[2] < gaps[1] < gaps[0] and gaps[-1] < 0.05 * gaps[0]
+    # fixed point of the PDHG map at convergence
+    x, y, info, _ = pdhg.solve(prob, steps="const", step_bound=0.124, term_maxiter=6000, granularity=6000)
+    tau = sigma = np.sqrt(0.124)
+    xn = prob.gprox(x - tau * prob.AT.dot(y), tau)
+    yn = prob.fconjprox(y + sigma * prob.A.dot(2 * xn - x), sigma)
+    assert np.abs(xn - x).max() < 1e-4 and np.abs(yn - y).max() < 1e-4
+
+
+def test_adaptive_rule_invariants():
+    H, W, lam = 10, 9, 3.0
+    f = _rects(H, W, 2).reshape(-1, 1)
+    prob = pdhg.tv_problem(f, (H, W), lam)
+    bnd = 0.124
+    x, y, info, hist = pdhg.solve(prob, steps="adaptive", step_bound=bnd, term_maxiter=300, granularity=10 ** 9,
+                                  history=True)
+    taus = np.array([h[0] for h in hist])
+    sigmas = np.array([h[1] for h in hist])
+    alphas = np.array([h[2] for h in hist])
+    # tau*sigma stays at the bound; alpha only shrinks by eta; steps change by (1-alpha)^{+-1}
+    np.testing.assert_allclose(taus * sigmas, bnd, rtol=1e-12)
+    ch = np.flatnonzero(np.diff(alphas) != 0)
+    np.testing.assert_allclose(alphas[ch + 1] / alphas[ch], 0.95, rtol=1e-12)
+    r = taus[ch + 1] / taus[ch]
+    ok = np.isclose(r, 1 / (1 - alphas[ch])) | np.isclose(r, 1 - alphas[ch])
+    assert ok.all() and len(ch) > 0
+
+
+def test_l1_problem_gap_certifies():
+    H, W, lam = 8, 8, 1.5
+    f = _rects(H, W, 3).reshape(-1, 1)
+    prob = pdhg.tv_problem(f, (H, W), lam, data="l1")
+    x, y, info, _ = pdhg.solve(prob, steps="const", step_bound=0.124, term_maxiter=3000, granularity=3000)
+    assert info["gap"] >= -1e-9 or info["infeasp"] > 0
+
+
+def test_truncate_examples():
+    # reference tools/__init__.py:6 examples
+    assert pdhg.truncate(0.006142, 3) == 0.00614
+    assert pdhg.truncate(2341.2, 3) == 2340.0
+    from opymize_amd.tools import truncate
+    for v in [0.006142, 2341.2, 0.12499, 0.5, 7.77777, 123456.0, -0.0412345]:
+        assert truncate(v, 3) == pdhg.truncate(v, 3)
