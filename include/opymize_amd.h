@@ -229,6 +229,77 @@ int opy_fn_eval_f32(int kind, const float* x, int64_t npts, int width, const opy
 int opy_fn_eval_f64(int kind, const double* x, int64_t npts, int width, const opy_fn_params_t* prm, double* work,
                     double* out3, int accumulate, void* stream);
 
+/* ---- fused, batched TV iterations ---------------------------------------
+ * PDHG for G + tvw*|grad x|_{2,1} with the forward-difference gradient on 2-D
+ * images of C <= 4 channels, many instances of uneven size per launch.  These
+ * entry points replace the chain of generic launches of one PDHG iteration
+ * (take_step, gprox, linop, overrelax, take_step, fconjprox, linop.adjoint,
+ * 4 residual reductions; reference pdhg.py:124-187) for that model. */
+#define OPY_TV_DATA_L2 0      /* G = 0.5*lam*|x - f|^2  (SSD, reference functionals/quad.py) */
+#define OPY_TV_DATA_L1 1      /* G = lam*|x - f|_1      (TV-L1 data term, new) */
+#define OPY_TV_STEPS_SCALAR 0 /* tau, sigma from the instance's opy_adapt_state_t (constant or adaptive) */
+#define OPY_TV_STEPS_PRECOND 1/* diagonal preconditioners of the stencil (reference pdhg.py:189-200) */
+
+typedef struct {
+  int64_t off;    /* element offset of the instance's x and f (pixels*C); y starts at 2*off */
+  int32_t H, W;
+  int32_t data;   /* OPY_TV_DATA_* */
+  int32_t tile0;  /* first per-tile partial slot of the instance */
+  int32_t ntiles; /* ceil(H/th)*ceil(W/tw) */
+  int32_t pad;
+  double lam;     /* data weight */
+  double tvw;     /* TV weight = radius of the dual ball */
+} opy_tv_inst_t;
+
+typedef struct {
+  int32_t C;      /* channels, 1..4 */
+  int32_t tw;     /* tile width; divides 256 */
+  int32_t rpt;    /* rows per thread; tile height th = 256/tw*rpt */
+  int32_t steps;  /* OPY_TV_STEPS_* */
+  double h0, h1;  /* grid spacing */
+  double w[4];    /* channel weights of the gradient */
+} opy_tv_cfg_t;
+
+typedef struct {
+  double objp, objd, gap, relgap, infeasp, infeasd;
+  double objd_cert, relgap_cert; /* dual value of the rescaled feasible dual point (L1) */
+  int32_t status;                /* 0 running, 1 converged, 2 non-finite */
+  int32_t iters;                 /* iterations run when last checked */
+} opy_tv_check_t;
+
+/* pass 1: x_out = prox_{tau G}(x_in - tau A^T y) on the tiles of the running instances */
+int opy_tv_primal_f32(const float* x_in, float* x_out, const float* y, const float* f, const opy_tv_inst_t* insts,
+                      const opy_adapt_state_t* steps, const int32_t* active_inst, const int32_t* act_off,
+                      const int32_t* nact, const opy_tv_cfg_t* cfg, int grid, void* stream);
+int opy_tv_primal_f64(const double* x_in, double* x_out, const double* y, const double* f, const opy_tv_inst_t* insts,
+                      const opy_adapt_state_t* steps, const int32_t* active_inst, const int32_t* act_off,
+                      const int32_t* nact, const opy_tv_cfg_t* cfg, int grid, void* stream);
+/* pass 2: y_out = proj(y_in + sigma A(2 x_new - x_old)); adapt=1 also writes 4 residual partials per tile */
+int opy_tv_dual_f32(const float* x_old, const float* x_new, const float* y_in, float* y_out,
+                    const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                    const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
+                    double* partials, int grid, void* stream);
+int opy_tv_dual_f64(const double* x_old, const double* x_new, const double* y_in, double* y_out,
+                    const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                    const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
+                    double* partials, int grid, void* stream);
+/* adaptive step rule per running instance from the residual partials (reference pdhg.py:155-183) */
+int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks,
+                 const double* partials, int B, int C, const opy_adapt_params_t* params, void* stream);
+/* FP64 gap partials per tile (reference pdhg.py:23-44) */
+int opy_tv_gap_f32(const float* x, const float* y, const float* f, const opy_tv_inst_t* insts,
+                   const int32_t* active_inst, const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg,
+                   double* partials, int grid, void* stream);
+int opy_tv_gap_f64(const double* x, const double* y, const double* f, const opy_tv_inst_t* insts,
+                   const int32_t* active_inst, const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg,
+                   double* partials, int grid, void* stream);
+/* per-instance gap, relative gap and termination test (reference pdhg.py:299-305) */
+int opy_tv_check(const opy_tv_inst_t* insts, opy_tv_check_t* checks, const double* partials, int B, double tol_gap,
+                 double tol_infeas, int certify, const int64_t* iter_counter, int iters_in_block, void* stream);
+/* rebuild the running-instance list (active_inst, act_off: B+1 entries) and advance the iteration counter */
+int opy_tv_compact(const opy_tv_inst_t* insts, const opy_tv_check_t* checks, int B, int32_t* active_inst,
+                   int32_t* act_off, int32_t* nact, int64_t* iter_counter, int iters_in_block, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

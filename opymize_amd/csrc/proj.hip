@@ -44,8 +44,11 @@ __global__ void __launch_bounds__(kBlock) l1proj_frob2_kernel(V2* __restrict__ x
 }
 
 // Spectral-ball projection via the 2x2 Gram matrix (one side of length 2).
+// The Gram matrix and its eigen-decomposition are formed in FP64 for both
+// element types: in FP32 the eigenvector formula (lmax - c11, c12) cancels
+// badly for nearly equal singular values.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) l1proj_spec_kernel(T* __restrict__ x, int64_t N, int M1, int M2, T lbd) {
+__global__ void __launch_bounds__(kBlock) l1proj_spec_kernel(T* __restrict__ x, int64_t N, int M1, int M2, double lbd) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   T* xi = x + i * (M1 * M2);
@@ -54,22 +57,22 @@ __global__ void __launch_bounds__(kBlock) l1proj_spec_kernel(T* __restrict__ x, 
   const int lim = two_rows ? M2 : M1;
   const int step_m = two_rows ? M1 : 1;   // stride between consecutive entries of a line
   const int step_j = two_rows ? 1 : M1;   // stride between the two lines
-  T c11 = 0, c12 = 0, c22 = 0;
+  double c11 = 0, c12 = 0, c22 = 0;
   for (int m = 0; m < lim; ++m) {
-    const T a0 = xi[m * step_m];
-    const T a1 = xi[m * step_m + step_j];
+    const double a0 = xi[m * step_m];
+    const double a1 = xi[m * step_m + step_j];
     c11 += a0 * a0;
     c12 += a0 * a1;
     c22 += a1 * a1;
   }
-  const T tr = c11 + c22;
-  const T d = sqrt(fmax(T(0), T(0.25) * tr * tr - (c11 * c22 - c12 * c12)));
-  const T lmax = fmax(T(0), T(0.5) * tr + d);
-  const T lmin = fmax(T(0), T(0.5) * tr - d);
-  const T smax = sqrt(lmax), smin = sqrt(lmin);
+  const double tr = c11 + c22;
+  const double d = sqrt(fmax(0.0, 0.25 * tr * tr - (c11 * c22 - c12 * c12)));
+  const double lmax = fmax(0.0, 0.5 * tr + d);
+  const double lmin = fmax(0.0, 0.5 * tr - d);
+  const double smax = sqrt(lmax), smin = sqrt(lmin);
   if (!(smax > lbd)) return;
-  T v11, v12, v21, v22;
-  if (c12 == T(0)) {
+  double v11, v12, v21, v22;
+  if (c12 == 0.0) {
     if (c11 >= c22) {
       v11 = 1; v12 = 0; v21 = 0; v22 = 1;
     } else {
@@ -80,24 +83,24 @@ __global__ void __launch_bounds__(kBlock) l1proj_spec_kernel(T* __restrict__ x, 
     v12 = c12;
     v21 = lmax - c11;
     v22 = lmin - c11;
-    T nv = hypot(v11, v21);
+    double nv = hypot(v11, v21);
     v11 /= nv;
     v21 /= nv;
     nv = hypot(v12, v22);
     v12 /= nv;
     v22 /= nv;
   }
-  const T s1 = fmin(smax, lbd) / smax;
-  const T s2 = smin > T(0) ? fmin(smin, lbd) / smin : T(0);
-  const T m11 = s1 * v11 * v11 + s2 * v12 * v12;
-  const T m12 = s1 * v11 * v21 + s2 * v12 * v22;
-  const T m21 = s1 * v21 * v11 + s2 * v22 * v12;
-  const T m22 = s1 * v21 * v21 + s2 * v22 * v22;
+  const double s1 = fmin(smax, lbd) / smax;
+  const double s2 = smin > 0.0 ? fmin(smin, lbd) / smin : 0.0;
+  const double m11 = s1 * v11 * v11 + s2 * v12 * v12;
+  const double m12 = s1 * v11 * v21 + s2 * v12 * v22;
+  const double m21 = s1 * v21 * v11 + s2 * v22 * v12;
+  const double m22 = s1 * v21 * v21 + s2 * v22 * v22;
   for (int m = 0; m < lim; ++m) {
-    const T a0 = xi[m * step_m];
-    const T a1 = xi[m * step_m + step_j];
-    xi[m * step_m] = a0 * m11 + a1 * m21;
-    xi[m * step_m + step_j] = a0 * m12 + a1 * m22;
+    const double a0 = xi[m * step_m];
+    const double a1 = xi[m * step_m + step_j];
+    xi[m * step_m] = (T)(a0 * m11 + a1 * m21);
+    xi[m * step_m + step_j] = (T)(a0 * m12 + a1 * m22);
   }
 }
 
@@ -173,7 +176,7 @@ int l1proj_impl(T* x, int64_t N, int M1, int M2, double lbd, int spectral, void*
   const bool vec = (M1 == 1 || M2 == 1);
   if (spectral && !vec) {
     if (M1 != 2 && M2 != 2) return fail(kInvalid, "l1norms_proj: spectral ball needs M1 or M2 == 2");
-    hipLaunchKernelGGL((l1proj_spec_kernel<T>), dim3(grid_for(N)), dim3(kBlock), 0, st, x, N, M1, M2, (T)lbd);
+    hipLaunchKernelGGL((l1proj_spec_kernel<T>), dim3(grid_for(N)), dim3(kBlock), 0, st, x, N, M1, M2, lbd);
   } else if (M1 * M2 == 2 && (reinterpret_cast<uintptr_t>(x) % (2 * sizeof(T)) == 0)) {
     using V2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
     hipLaunchKernelGGL((l1proj_frob2_kernel<T, V2>), dim3(grid_for(N)), dim3(kBlock), 0, st,

@@ -15,14 +15,18 @@ def _chk(x, y, nx, ny, what):
 
 class MatrixMult(LinOp):
     """Label-major product y_ji = sum_k A_jk x_ki (trans: A_kj)
-    (reference: einsum.py:17-56)."""
+    (reference: einsum.py:17-56).
+
+    The reference sizes x as (A.shape[0], N) and y as (A.shape[1], N), which
+    only agrees with its own formula for square A; here x is (K, N) and y is
+    (J, N) for the (J, K) matrix of the formula, identical for square A."""
 
     def __init__(self, N, A, trans=False, adjoint=None):
         LinOp.__init__(self)
         A = np.asarray(A, dtype=np.float64)
-        j, k = (1, 0) if trans else (0, 1)
-        self.x = Variable((A.shape[j], N))
-        self.y = Variable((A.shape[k], N))
+        k, j = (0, 1) if trans else (1, 0)
+        self.x = Variable((A.shape[k], N))
+        self.y = Variable((A.shape[j], N))
         self.N = int(N)
         self.trans = trans
         self.A = A

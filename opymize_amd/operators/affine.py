@@ -109,7 +109,9 @@ class ShiftScaleOp(Operator):
         sa, ss = self._arg("shift", self.shift, x.dtype)
         aa, as_ = self._arg("a", self.a, x.dtype)
         ba, bs = self._arg("b", self.b, x.dtype)
-        if self.b is 0 or (not isinstance(self.shift, (np.ndarray, torch.Tensor)) and float(self.shift) == 0):
+        b_zero = not isinstance(self.b, (np.ndarray, torch.Tensor)) and float(self.b) == 0
+        s_zero = not isinstance(self.shift, (np.ndarray, torch.Tensor)) and float(self.shift) == 0
+        if b_zero or s_zero:
             ba, bs = None, 0.0
         _hip.call("shiftscale", x.dtype, _hip.ptr(x), _hip.ptr(y), x.numel(), sa, ss, aa, as_, ba, bs, int(add),
                   _hip.stream())

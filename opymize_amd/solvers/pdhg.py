@@ -34,6 +34,39 @@ def _device_step_ok(F):
     return bool(getattr(F, "device_step", False))
 
 
+def _as_host(v):
+    return v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else np.asarray(v)
+
+
+def fused_tv_model(g, f, A):
+    """The TV-L2 / TV-L1 denoising structure the fused kernels implement, or None:
+    G = SSD(data, vol) with equal volumes, no mask and no shift, or L1Data with
+    equal weights; F = L1Norms(N, M, lbd) with the Frobenius norm and
+    M1*M2 = 2*C; A = GradientOp(2-D, C <= 4, scheme="forward")."""
+    from opymize_amd.functionals import SSD, L1Data, L1Norms
+    from opymize_amd.linear import GradientOp
+    if not isinstance(A, GradientOp) or A.scheme != "forward" or len(A.imagedims) != 2 or A.nchannels > 4:
+        return None
+    C = A.nchannels
+    N = int(np.prod(A.imagedims))
+    if not isinstance(f, L1Norms) or f.matrixnorm != "frobenius" or f.N != N or f.M[0] * f.M[1] != 2 * C:
+        return None
+    if isinstance(g, SSD):
+        if not g.mask.all() or g.shift != 0 or not np.all(g.vol == g.vol[0]):
+            return None
+        data, lam = "l2", float(g.vol[0])
+    elif isinstance(g, L1Data):
+        if not np.all(g.lbd == g.lbd[0]):
+            return None
+        data, lam = "l1", float(g.lbd[0])
+    else:
+        return None
+    if g.f.shape != (N, C):
+        return None
+    H, W = A.imagedims
+    return {"data": data, "lam": lam, "tvw": f.lbd, "image": g.f.reshape(H, W, C), "h": A.imageh, "w": A.weights}
+
+
 class GapEvaluator(object):
     """P = G(x) + F(Ax) and D = -G*(-A^T y) - F*(y) in FP64 on the device
     with one device-to-host copy (reference: pdhg.py:23-44)."""
@@ -260,6 +293,13 @@ class PDHG(object):
         if not use_gpu:
             raise ValueError("opymize_amd implements the GPU path of PDHG (use_gpu=True)")
         _hip.require_device()
+        if fused and term_pd_res is None and term_maxiter > 0:
+            model = fused_tv_model(self.g, self.f, self.linop)
+            if model is not None:
+                return self._solve_fused(model, continue_at, precision, steps, step_bound, step_factor,
+                                         term_pd_gap, term_maxiter, granularity, cbfun, use_graph)
+            if fused is True:
+                raise ValueError("fused=True but the model is not a 2-D TV-L2/TV-L1 problem")
         i, c, d = self.itervars, self.constvars, self._dev
         dtype = _hip.torch_dtype(precision)
         npdt = _hip.np_dtype(dtype)
@@ -361,6 +401,64 @@ class PDHG(object):
                         cbfun(self.state, self.info)
                     if test_term or interrupt_hdl.interrupted:
                         break
+        torch.cuda.synchronize()
+        return self.info
+
+    def _solve_fused(self, model, continue_at, precision, steps, step_bound, step_factor, term_pd_gap,
+                     term_maxiter, granularity, cbfun, use_graph):
+        """Same loop semantics as solve(), through the fused TV kernels."""
+        from opymize_amd.solvers.tv import TVBatch
+        from opymize_amd.solvers import autotune
+        if not isinstance(term_pd_gap, tuple):
+            term_pd_gap = (term_pd_gap, term_pd_gap)
+        if steps not in ("precond", "adaptive"):
+            steps_mode = "const"
+        else:
+            steps_mode = steps
+        bnd = step_bound
+        if steps_mode != "precond" and bnd is None:
+            logging.info("Estimating optimal step bound...")
+            op_norm, _ = normest(self.linop)
+            bnd = truncate(1.0 / op_norm ** 2, 3)
+        H, W, C = model["image"].shape
+        tile = autotune.default_tile(H, W, C)
+        eng = TVBatch([model["image"]], model["lam"], model["data"], model["tvw"], precision=precision,
+                      tile=tile, h=model["h"], weights=model["w"])
+        if continue_at is not None:
+            x0, y0 = continue_at
+            eng.set_start(eng.pack([_as_host(x0)], "x"), eng.pack([_as_host(y0)], "y"))
+        if granularity % 50 == 0:
+            block, every = 50, granularity // 50
+        else:
+            block, every = granularity, 1
+        self.constvars = {"theta": 1.0, "fused": True}
+        if steps_mode == "adaptive":
+            self.constvars.update(adaptive=True, eta=0.95, Delta=1.5)
+        elif steps_mode == "precond":
+            self.constvars["precond"] = True
+        logging.info("Fused TV kernels (%s data term, %s steps, tile %dx%d)"
+                     % (model["data"], steps_mode, eng.tw, eng.rpt))
+        self.log_header(False)
+        self._fused = eng
+        with GracefulInterruptHandler() as interrupt_hdl:
+            def cb(e, it):
+                r = e.results()[0]
+                self.info.update({k: r[k] for k in ("objp", "objd", "infeasp", "infeasd", "gap", "relgap")})
+                self.info["iter"] = it
+                if steps_mode == "adaptive":
+                    s = e.steps_state()[0]
+                    for k in ("resp", "resd", "epsp", "epsd"):
+                        self.info[k] = s[k]
+                    self.itervars.update(tauk=s["tau"], sigmak=s["sigma"], alphak=s["alpha"])
+                xs, ys = e.solution()
+                self.itervars["xk"], self.itervars["yk"] = xs[0], ys[0]
+                self.log_iter(False)
+                if cbfun is not None:
+                    cbfun(self.state, self.info)
+                return interrupt_hdl.interrupted
+            eng.solve(steps=steps_mode, step_bound=bnd if bnd is not None else 1.0, step_factor=step_factor,
+                      tol=term_pd_gap[0], tol_infeas=term_pd_gap[1], max_iters=term_maxiter, block_iters=block,
+                      check_every=every, certify=False, use_graph=use_graph, callback=cb)
         torch.cuda.synchronize()
         return self.info
 

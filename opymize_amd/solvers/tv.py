@@ -1,0 +1,375 @@
+"""Fused, batched PDHG for TV-regularised denoising of 2-D images.
+
+One engine serves both the drop-in single-instance path (PDHG.solve on a
+recognised TV-L2/TV-L1 model) and the batched solver over many instances of
+uneven size.  Per iteration it launches two fused passes (and, with adaptive
+steps, one small per-instance rule kernel); every `block_iters` iterations it
+evaluates the FP64 primal-dual gap, marks converged instances and compacts
+the list of running instances -- all on the device, so a whole block is one
+hipGraph replay and the host only reads the number of running instances.
+
+Model of instance b: min_x G_b(x) + tvw_b * sum_p |grad x(p)|_F with
+G_b = 0.5*lam_b*|x - f_b|^2 (data="l2") or lam_b*|x - f_b|_1 (data="l1"),
+the forward-difference gradient of the reference (diff.cu:64-71) and the
+PDHG iteration of the reference (pdhg.py:124-187).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from opymize_amd import _hip
+
+_DATA = {"l2": 0, "l1": 1}
+ALIGN = 64  # elements; every instance starts on a 256-byte boundary (FP32)
+
+
+class TVInst(ctypes.Structure):
+    _fields_ = [("off", ctypes.c_int64), ("H", ctypes.c_int32), ("W", ctypes.c_int32),
+                ("data", ctypes.c_int32), ("tile0", ctypes.c_int32), ("ntiles", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("lam", ctypes.c_double), ("tvw", ctypes.c_double)]
+
+
+class TVCfg(ctypes.Structure):
+    _fields_ = [("C", ctypes.c_int32), ("tw", ctypes.c_int32), ("rpt", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("h0", ctypes.c_double), ("h1", ctypes.c_double), ("w", ctypes.c_double * 4)]
+
+
+class TVCheck(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("objp", "objd", "gap", "relgap", "infeasp", "infeasd", "objd_cert", "relgap_cert")] + \
+               [("status", ctypes.c_int32), ("iters", ctypes.c_int32)]
+
+
+CHECK_DOUBLES = ctypes.sizeof(TVCheck) // 8
+STATE_DOUBLES = _hip.ADAPT_STATE_DOUBLES
+assert ctypes.sizeof(TVInst) == 48 and ctypes.sizeof(TVCfg) == 64 and ctypes.sizeof(TVCheck) == 72
+
+_SIGS = {
+    "opy_tv_primal": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_dual": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_gap": [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p],
+}
+_PLAIN = {
+    "opy_tv_adapt": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+    "opy_tv_check": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_compact": [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int,
+                                                                                      ctypes.c_void_p],
+}
+_bound = False
+
+
+def _bind():
+    global _bound
+    if _bound:
+        return
+    lib = _hip.load()
+    for k, a in _SIGS.items():
+        for s in ("_f32", "_f64"):
+            fn = getattr(lib, k + s)
+            fn.restype, fn.argtypes = ctypes.c_int, a
+    for k, a in _PLAIN.items():
+        fn = getattr(lib, k)
+        fn.restype, fn.argtypes = ctypes.c_int, a
+    _bound = True
+
+
+def exported_names():
+    return [k + s for k in _SIGS for s in ("_f32", "_f64")] + list(_PLAIN)
+
+
+def _cu_count():
+    return torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+
+
+def tile_height(tw, rpt):
+    return (256 // tw) * rpt
+
+
+class TVBatch(object):
+    """Batched fused TV PDHG.
+
+    images: list of arrays (H, W) or (H, W, C) (all with the same C <= 4)
+    lam:    data weight per instance (scalar or list)
+    data:   "l2" / "l1" per instance (str or list)
+    tvw:    TV weight per instance (scalar or list)
+    """
+
+    def __init__(self, images, lam, data="l2", tvw=1.0, precision="float", tile=(64, 4), h=(1.0, 1.0),
+                 weights=None, grid=None):
+        _hip.require_device()
+        _bind()
+        self.B = len(images)
+        if self.B < 1:
+            raise ValueError("need at least one instance")
+        imgs = [np.asarray(im, dtype=np.float64) for im in images]
+        imgs = [im[:, :, None] if im.ndim == 2 else im for im in imgs]
+        C = imgs[0].shape[2]
+        if any(im.shape[2] != C for im in imgs) or not 1 <= C <= 4:
+            raise ValueError("all instances need the same channel count C in [1, 4]")
+        self.C = C
+        self.shapes = [im.shape[:2] for im in imgs]
+        lam = np.broadcast_to(np.asarray(lam, dtype=np.float64), (self.B,))
+        tvw = np.broadcast_to(np.asarray(tvw, dtype=np.float64), (self.B,))
+        data = [data] * self.B if isinstance(data, str) else list(data)
+        if np.any(lam <= 0) or np.any(tvw <= 0):
+            raise ValueError("lam and tvw must be positive")
+        self.lam, self.tvw, self.data = lam.copy(), tvw.copy(), data
+        self.dtype = _hip.torch_dtype(precision)
+        self.h = (float(h[0]), float(h[1]))
+        self.weights = np.ones(C) if weights is None else np.asarray(weights, dtype=np.float64)
+        # packed layout
+        offs, o = [], 0
+        for (H, W) in self.shapes:
+            offs.append(o)
+            o += ((H * W * C + ALIGN - 1) // ALIGN) * ALIGN
+        self.offs = offs
+        self.n_total = o
+        fpack = np.zeros(o)
+        for im, off in zip(imgs, offs):
+            fpack[off:off + im.size] = im.ravel()
+        self.f = _hip.to_device(fpack, self.dtype)
+        self.x = [torch.zeros(o, dtype=self.dtype, device="cuda") for _ in range(2)]
+        self.y = [torch.zeros(2 * o, dtype=self.dtype, device="cuda") for _ in range(2)]
+        self._grid_override = grid
+        self.set_tile(*tile)
+        self.graph = None
+
+    # -- layout --------------------------------------------------------------
+    def set_tile(self, tw, rpt):
+        th = tile_height(tw, rpt)
+        self.tw, self.rpt = int(tw), int(rpt)
+        insts = (TVInst * self.B)()
+        t0 = 0
+        self.ntiles = []
+        for b, (H, W) in enumerate(self.shapes):
+            nt = ((H + th - 1) // th) * ((W + tw - 1) // tw)
+            I = insts[b]
+            I.off, I.H, I.W = self.offs[b], H, W
+            I.data = _DATA[self.data[b]]
+            I.tile0, I.ntiles = t0, nt
+            I.lam, I.tvw = float(self.lam[b]), float(self.tvw[b])
+            self.ntiles.append(nt)
+            t0 += nt
+        self.total_tiles = t0
+        raw = np.frombuffer(bytes(insts), dtype=np.uint8).copy()
+        self.insts = torch.as_tensor(raw).cuda()
+        self.part_res = torch.zeros(4 * t0, dtype=torch.float64, device="cuda")
+        self.part_gap = torch.zeros(6 * t0, dtype=torch.float64, device="cuda")
+        self.active_inst = torch.zeros(self.B, dtype=torch.int32, device="cuda")
+        self.act_off = torch.zeros(self.B + 1, dtype=torch.int32, device="cuda")
+        self.nact = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.iter_counter = torch.zeros(1, dtype=torch.int64, device="cuda")
+        g = self._grid_override or 8 * _cu_count()
+        self.grid = int(max(1, min(g, t0)))
+        self.graph = None
+
+    def _cfg(self, steps):
+        c = TVCfg()
+        c.C, c.tw, c.rpt = self.C, self.tw, self.rpt
+        c.steps = 1 if steps == "precond" else 0
+        c.h0, c.h1 = self.h
+        for k in range(4):
+            c.w[k] = float(self.weights[k]) if k < self.C else 1.0
+        return c
+
+    # -- state -----------------------------------------------------------------
+    def set_start(self, x0=None, y0=None):
+        """Packed start values (device tensors or host arrays of the packed size), zeros otherwise."""
+        for t, v in ((self.x[0], x0), (self.y[0], y0)):
+            if v is None:
+                t.zero_()
+            else:
+                t.copy_(v.reshape(-1).to(self.dtype) if _hip.is_device_tensor(v)
+                        else torch.as_tensor(np.asarray(v, dtype=_hip.np_dtype(self.dtype)).ravel()))
+
+    def pack(self, arrays, field="x"):
+        """Host list of per-instance arrays -> packed host vector."""
+        mult = 1 if field == "x" else 2
+        out = np.zeros(mult * self.n_total)
+        for a, off in zip(arrays, self.offs):
+            a = np.asarray(a, dtype=np.float64).ravel()
+            out[mult * off:mult * off + a.size] = a
+        return out
+
+    def unpack(self, t, field="x"):
+        mult = 1 if field == "x" else 2
+        h = t.cpu().numpy().astype(np.float64)
+        res = []
+        for (H, W), off in zip(self.shapes, self.offs):
+            n = H * W * self.C * mult
+            res.append(h[mult * off:mult * off + n].copy())
+        return res
+
+    def _reset_state(self, steps, step_bound, step_factor, tol, tol_infeas, certify, statuses=None):
+        st = np.zeros((self.B, STATE_DOUBLES))
+        F = _hip.ADAPT_FIELD
+        bnd = step_bound
+        if steps == "adaptive":
+            st[:, F["tau"]] = st[:, F["sigma"]] = np.sqrt(bnd)
+            st[:, F["alpha"]] = 0.5
+        else:
+            sigma = np.sqrt(bnd / step_factor)
+            st[:, F["sigma"]] = sigma
+            st[:, F["tau"]] = bnd / sigma
+        self.state = torch.as_tensor(st).cuda()
+        chk = np.zeros((self.B, CHECK_DOUBLES))
+        if statuses is not None:
+            chk.view(np.int32)[:, 2 * 8] = statuses
+        self.checks = torch.as_tensor(chk).cuda()
+        self.iter_counter.zero_()
+        self._compact(0)
+
+    # -- kernels ---------------------------------------------------------------
+    def _sfx(self):
+        return "_f32" if self.dtype == torch.float32 else "_f64"
+
+    def _call(self, name, *args):
+        lib = _hip.load()
+        fn = getattr(lib, name)
+        rc = fn(*args)
+        if rc != 0:
+            raise _hip.HipError("%s failed: %s" % (name, lib.opy_last_error().decode()))
+
+    def _iteration(self, p, cfg, adapt, params):
+        P = _hip.ptr
+        s = _hip.stream()
+        xo, xn = self.x[p], self.x[1 - p]
+        yo, yn = self.y[p], self.y[1 - p]
+        self._call("opy_tv_primal" + self._sfx(), P(xo), P(xn), P(yo), P(self.f), P(self.insts), P(self.state),
+                   P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), self.grid, s)
+        self._call("opy_tv_dual" + self._sfx(), P(xo), P(xn), P(yo), P(yn), P(self.insts), P(self.state),
+                   P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), int(adapt),
+                   P(self.part_res), self.grid, s)
+        if adapt:
+            self._call("opy_tv_adapt", P(self.insts), P(self.state), P(self.checks), P(self.part_res), self.B,
+                       self.C, ctypes.byref(params), s)
+
+    def _gap_check(self, cfg, tol, tol_infeas, certify, iters_in_block):
+        P = _hip.ptr
+        s = _hip.stream()
+        self._call("opy_tv_gap" + self._sfx(), P(self.x[0]), P(self.y[0]), P(self.f), P(self.insts),
+                   P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), P(self.part_gap),
+                   self.grid, s)
+        self._call("opy_tv_check", P(self.insts), P(self.checks), P(self.part_gap), self.B, float(tol),
+                   float(tol_infeas), int(certify), P(self.iter_counter), int(iters_in_block), s)
+
+    def _compact(self, iters_in_block):
+        P = _hip.ptr
+        self._call("opy_tv_compact", P(self.insts), P(self.checks), self.B, P(self.active_inst), P(self.act_off),
+                   P(self.nact), P(self.iter_counter), int(iters_in_block), _hip.stream())
+
+    def _iterations(self, n, cfg, adapt, params):
+        """n iterations starting from buffer self.cur (updates self.cur)."""
+        for _ in range(n):
+            self._iteration(self.cur, cfg, adapt, params)
+            self.cur ^= 1
+
+    def _check_step(self, cfg, tol, tol_infeas, certify, iters_since):
+        self._gap_check(cfg, tol, tol_infeas, certify, iters_since)
+        self._compact(iters_since)
+
+    @staticmethod
+    def _capture(fn):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                fn()
+        torch.cuda.current_stream().wait_stream(s)
+        return g
+
+    # -- public ----------------------------------------------------------------
+    def check_now(self, steps="adaptive", tol=1e-5, tol_infeas=1e-5, certify=True):
+        """Gap of the current iterates of the running instances (no iteration)."""
+        cfg = self._cfg(steps)
+        self._gap_check(cfg, tol, tol_infeas, certify, 0)
+        self._compact(0)
+        return self.results()
+
+    def solve(self, steps="adaptive", step_bound=0.124, step_factor=1.0, tol=1e-5, tol_infeas=None,
+              max_iters=20000, block_iters=50, check_every=1, certify=True, use_graph=True, eps_res=None,
+              callback=None, check_start=False, keep_state=False):
+        """Iterate until every instance is certified (|relgap| < tol and
+        infeasibility < tol_infeas) or max_iters; the test runs every
+        block_iters*check_every iterations.  Returns self.results()."""
+        tol_infeas = tol if tol_infeas is None else tol_infeas
+        adapt = steps == "adaptive"
+        cfg = self._cfg(steps)
+        params = _hip.AdaptParams()
+        e = tol if eps_res is None else eps_res
+        params.eta, params.Delta = 0.95, 1.5
+        params.eps_absp = params.eps_relp = params.eps_absd = params.eps_reld = e
+        if not keep_state:
+            self._reset_state(steps, step_bound, step_factor, tol, tol_infeas, certify)
+        self.cur = 0
+        if check_start:
+            self._check_step(cfg, tol, tol_infeas, certify, 0)
+            if int(self.nact.item()) == 0:
+                self.iters_run = 0
+                return self.results()
+        # graphs need an even block (the buffer roles return to their start)
+        graphs = use_graph and block_iters % 2 == 0
+        g_it = g_ck = None
+        done_iters, nblk = 0, 0
+        since = 0
+        while done_iters < max_iters:
+            n = min(block_iters, max_iters - done_iters)
+            if graphs and n == block_iters:
+                if g_it is None:
+                    self._iterations(n, cfg, adapt, params)  # eager warm-up block, counted
+                    torch.cuda.synchronize()
+                    g_it = self._capture(lambda: self._iterations(n, cfg, adapt, params))
+                else:
+                    g_it.replay()
+            else:
+                self._iterations(n, cfg, adapt, params)
+            done_iters += n
+            since += n
+            nblk += 1
+            if nblk % check_every == 0 or done_iters >= max_iters:
+                if self.cur != 0:
+                    # odd number of iterations: bring the current iterate to buffer 0
+                    for a in (self.x, self.y):
+                        _hip.call("copy", self.dtype, _hip.ptr(a[1]), _hip.ptr(a[0]), a[0].numel(), _hip.stream())
+                    self.cur = 0
+                if graphs and since == block_iters * check_every:
+                    if g_ck is None:
+                        self._check_step(cfg, tol, tol_infeas, certify, since)
+                        torch.cuda.synchronize()
+                        g_ck = self._capture(lambda: self._check_step(cfg, tol, tol_infeas, certify,
+                                                                      block_iters * check_every))
+                    else:
+                        g_ck.replay()
+                else:
+                    self._check_step(cfg, tol, tol_infeas, certify, since)
+                since = 0
+                nrun = int(self.nact.item())
+                stop = callback(self, done_iters) if callback is not None else False
+                if nrun == 0 or stop:
+                    break
+        self.iters_run = done_iters
+        return self.results()
+
+    def results(self):
+        chk = self.checks.cpu().numpy()
+        out = []
+        iv = chk.view(np.int32)
+        for b in range(self.B):
+            c = chk[b]
+            out.append({"objp": c[0], "objd": c[1], "gap": c[2], "relgap": c[3], "infeasp": c[4],
+                        "infeasd": c[5], "objd_cert": c[6], "relgap_cert": c[7],
+                        "status": int(iv[b, 16]), "iter": int(iv[b, 17])})
+        return out
+
+    def steps_state(self):
+        st = self.state.cpu().numpy()
+        F = _hip.ADAPT_FIELD
+        return [{k: float(st[b, F[k]]) for k in ("tau", "sigma", "alpha", "resp", "resd", "epsp", "epsd")}
+                for b in range(self.B)]
+
+    def solution(self):
+        """Per-instance (x, y) host arrays of the current iterates (buffer 0)."""
+        return self.unpack(self.x[0], "x"), self.unpack(self.y[0], "y")

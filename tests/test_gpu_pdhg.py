@@ -31,13 +31,14 @@ def tv_model(f, dims, lam, data="l2", tv_lbd=1.0):
     return G, F, A
 
 
-def test_known_answer_three_iterations(gpu):
+@pytest.mark.parametrize("fused", [False, True])
+def test_known_answer_three_iterations(gpu, fused):
     from opymize_amd.solvers import PDHG
     f = np.array([[0.0], [1.0]])
     G, F, A = tv_model(f, (1, 2), 1.0)
     for k, (xe, ye) in {1: ([0, 1 / 3], 1 / 3), 2: ([1 / 9, 4 / 9], 1 / 2), 3: ([13 / 54, 25 / 54], 5 / 9)}.items():
         s = PDHG(G, F, A)
-        s.solve(steps="const", step_bound=0.25, term_maxiter=k, granularity=k, use_graph=False)
+        s.solve(steps="const", step_bound=0.25, term_maxiter=k, granularity=k, use_graph=False, fused=fused)
         x, y = s.state
         np.testing.assert_allclose(x, xe, atol=1e-15)
         np.testing.assert_allclose(y, [0, ye, 0, 0], atol=1e-15)
@@ -46,7 +47,8 @@ def test_known_answer_three_iterations(gpu):
 @pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("steps", ["const", "precond", "adaptive"])
 @pytest.mark.parametrize("data", ["l2", "l1"])
-def test_tv_iterates_match_oracle(gpu, precision, steps, data):
+@pytest.mark.parametrize("fused", [False, True])
+def test_tv_iterates_match_oracle(gpu, precision, steps, data, fused):
     from opymize_amd.solvers import PDHG
     H, W, C, lam = 20, 23, 3 if data == "l2" else 1, 4.0
     f = rects(H, W, C, 1)
@@ -54,7 +56,8 @@ def test_tv_iterates_match_oracle(gpu, precision, steps, data):
     k = 60
     s = PDHG(G, F, A)
     info = s.solve(steps=steps, precision=precision, step_bound=0.124, term_maxiter=k, granularity=k,
-                   term_pd_gap=1e-30)
+                   term_pd_gap=1e-30, fused=fused)
+    assert (s._fused is not None) == fused
     x, y = s.state
     prob = opd.tv_problem(f, (H, W), lam, data)
     xo, yo, io, _ = opd.solve(prob, steps=steps, step_bound=0.124, term_maxiter=k, granularity=k,
@@ -68,7 +71,8 @@ def test_tv_iterates_match_oracle(gpu, precision, steps, data):
         assert abs(s.itervars["tauk"] - io["tau"]) <= 1e-9 * io["tau"]
 
 
-def test_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("fused", [False, True])
+def test_graph_replays_match_eager(gpu, fused):
     """The hipGraph of an iteration pair gives bitwise the eager iterates, on
     every replay, with eager work in between."""
     from opymize_amd.solvers import PDHG
@@ -84,7 +88,7 @@ def test_graph_replays_match_eager(gpu):
             outs.append((state[0].copy(), info["gap"]))
 
         s.solve(steps="adaptive", step_bound=0.124, term_maxiter=40, granularity=10, term_pd_gap=1e-30,
-                use_graph=use_graph, cbfun=cb)
+                use_graph=use_graph, cbfun=cb, fused=fused)
         res[use_graph] = outs
     assert len(res[True]) == len(res[False]) == 4
     for (xa, ga), (xb, gb) in zip(res[False], res[True]):
@@ -92,14 +96,15 @@ def test_graph_replays_match_eager(gpu):
         assert ga == gb
 
 
-def test_converges_and_certifies(gpu):
+@pytest.mark.parametrize("fused", [False, True])
+def test_converges_and_certifies(gpu, fused):
     from opymize_amd.solvers import PDHG
     H, W, lam = 48, 40, 8.0
     f = rects(H, W, 1, 3)
     G, F, A = tv_model(f, (H, W), lam)
     s = PDHG(G, F, A)
     info = s.solve(steps="adaptive", precision="float", step_bound=0.124, term_maxiter=20000, granularity=50,
-                   term_pd_gap=1e-5)
+                   term_pd_gap=1e-5, fused=fused)
     assert abs(info["relgap"]) < 1e-5 and info["iter"] < 20000
     prob = opd.tv_problem(f, (H, W), lam)
     xo, yo, io, _ = opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=20000, granularity=50,

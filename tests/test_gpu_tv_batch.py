@@ -1,0 +1,101 @@
+"""Batched fused TV PDHG: every instance of an uneven batch against its own
+oracle run.  Tolerances as in test_gpu_pdhg.py (FP64 1e-9, FP32 5e-4 max abs)."""
+import numpy as np
+import pytest
+
+from oracle import pdhg as opd
+
+pytestmark = pytest.mark.gpu
+
+
+def rects(H, W, seed, nrect=6):
+    rng = np.random.default_rng(seed)
+    img = np.full((H, W), 0.5)
+    for _ in range(nrect):
+        y0, y1 = sorted(rng.integers(0, H, 2))
+        x0, x1 = sorted(rng.integers(0, W, 2))
+        img[y0:y1 + 1, x0:x1 + 1] = rng.random()
+    return img + 0.1 * rng.standard_normal((H, W))
+
+
+SHAPES = [(17, 30), (40, 23), (5, 64), (64, 5), (1, 33), (70, 70)]
+
+
+def make_batch(precision, tile=(64, 2)):
+    from opymize_amd.solvers.tv import TVBatch
+    imgs = [rects(H, W, i) for i, (H, W) in enumerate(SHAPES)]
+    lams = [2.0, 7.5, 1.0, 15.0, 4.0, 3.3]
+    data = ["l2", "l1", "l2", "l1", "l2", "l2"]
+    return TVBatch(imgs, lams, data, precision=precision, tile=tile), imgs, lams, data
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("steps", ["adaptive", "const", "precond"])
+@pytest.mark.parametrize("tile", [(64, 2), (32, 1), (16, 4)])
+def test_batch_iterates_match_oracle(gpu, precision, steps, tile):
+    bat, imgs, lams, data = make_batch(precision, tile)
+    k = 40
+    bat.solve(steps=steps, step_bound=0.124, max_iters=k, block_iters=20, tol=1e-30)
+    xs, ys = bat.solution()
+    res = bat.results()
+    tol = 1e-9 if precision == "double" else 5e-4
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps=steps, step_bound=0.124, term_maxiter=k, granularity=k,
+                                  term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < tol, b
+        assert np.abs(ys[b] - yo).max() < tol, b
+        rel = 1e-9 if precision == "double" else 1e-4
+        assert abs(res[b]["gap"] - io["gap"]) <= rel * max(1.0, abs(io["objp"])), b
+        assert res[b]["iter"] == k and res[b]["status"] == 0
+    if steps == "adaptive":
+        st = bat.steps_state()
+        for b, im in enumerate(imgs):
+            H, W = im.shape
+            prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+            _, _, io, _ = opd.solve(prob, steps=steps, step_bound=0.124, term_maxiter=k, granularity=k,
+                                    term_pd_gap=1e-30)
+            assert abs(st[b]["tau"] - io["tau"]) <= 1e-9 * io["tau"], b
+
+
+def test_batch_converges_certified_and_drops_instances(gpu):
+    bat, imgs, lams, data = make_batch("float")
+    seen = []
+    res = bat.solve(steps="adaptive", step_bound=0.124, max_iters=20000, block_iters=50, tol=1e-5,
+                    callback=lambda e, it: seen.append(int(e.nact.item())))
+    assert all(r["status"] == 1 for r in res), res
+    assert all(abs(r["relgap_cert"]) < 1e-5 for r in res)
+    # instances leave the running set as they converge
+    assert seen == sorted(seen, reverse=True) and seen[-1] == 0
+    xs, _ = bat.solution()
+    for b, im in enumerate(imgs):
+        if data[b] != "l2":
+            continue
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b])
+        xo, _, io, _ = opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=30000, granularity=50,
+                                 term_pd_gap=1e-10)
+        bound = np.sqrt(2 * max(res[b]["gap"], 0) / lams[b]) + np.sqrt(2 * max(io["gap"], 0) / lams[b])
+        assert np.linalg.norm(xs[b] - xo) <= 1.01 * bound + 1e-6, b
+
+
+def test_batch_graph_equals_eager(gpu):
+    out = {}
+    for g in (False, True):
+        bat, *_ = make_batch("double")
+        bat.solve(steps="adaptive", step_bound=0.124, max_iters=200, block_iters=20, tol=1e-30, use_graph=g)
+        out[g] = bat.solution()[0]
+    for a, b in zip(out[False], out[True]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_batch_warm_start_from_solution(gpu):
+    """Continuing from a converged solution certifies at the first check."""
+    bat, imgs, lams, data = make_batch("double")
+    bat.solve(steps="adaptive", step_bound=0.124, max_iters=20000, tol=1e-7)
+    x, y = bat.x[0].clone(), bat.y[0].clone()
+    bat2, *_ = make_batch("double")
+    bat2.set_start(x, y)
+    res = bat2.solve(steps="adaptive", step_bound=0.124, max_iters=1000, tol=1e-6, check_start=True)
+    assert all(r["status"] == 1 for r in res) and all(r["iter"] == 0 for r in res)
