@@ -9,19 +9,26 @@
 // reference (pdhg.py:124-187, diff.cu:64-71, :114-123, proj.cu:34-44).
 //
 // One iteration is two passes over HBM:
-//   pass 1 (primal): x+ = prox_{tau G}(x - tau * A^T y)      reads x, y, f; writes x+
+//   pass 1 (primal): x+ = prox_{tau G}(x - tau * A^T y)           reads x, y, f; writes x+
 //   pass 2 (dual):   y+ = proj_{|.|<=tvw}(y + sigma * A(2x+ - x))  reads x, x+, y; writes y+
 // The over-relaxed A x_bar = 2 A x+ - A x of the reference is applied as
 // A(2x+ - x) (equal by linearity).  With adaptive steps pass 2 also forms the
-// four residual norms of pdhg.py:155-164 by recomputing y+ on a one-pixel halo
-// (so that A^T y+ is available) -- no extra sweep over memory.
+// four residual norms of pdhg.py:155-164.
 //
-// Work is split into tiles of tw x (256/tw*rpt) pixels.  A persistent grid
-// walks the tiles of the still-running instances: active_inst / act_off hold
-// the running instances and the exclusive prefix of their tile counts, a tile
-// index is mapped to its instance by binary search.  All per-instance
-// reductions (residuals, gap) go through per-tile FP64 partials that a
-// per-instance kernel folds in a fixed order, so results are reproducible.
+// Thread layout: a tile is tw columns x (256/tw)*RPT rows; a thread owns one
+// column segment of RPT consecutive rows and marches down it, carrying the
+// row above / below in registers.  Horizontal neighbours come from the
+// neighbouring lane by cross-lane shuffles; only the first/last lane of a row
+// segment of a wavefront loads (or, for y+, recomputes) its neighbour.
+//
+// A persistent grid walks the tiles of the still-running instances:
+// active_inst / act_off hold the running instances and the exclusive prefix
+// of their tile counts, a tile index is mapped to its instance by binary
+// search.  Blocks are renumbered so that consecutive tiles run on the same
+// XCD (the dispatcher deals blocks round-robin to the 8 XCDs, each with its
+// own L2), which keeps the halo rows of neighbouring tiles in one L2.  All
+// per-instance reductions (residuals, gap) go through per-tile FP64 partials
+// that a per-instance kernel folds in a fixed order (reproducible results).
 #include "opy_common.h"
 
 #include <cmath>
@@ -30,6 +37,7 @@ namespace opy {
 namespace {
 
 constexpr int kTB = 256;
+constexpr double kEps = 2.220446049250313e-16;
 
 struct Cfg {
   int C, tw, rpt, steps;
@@ -50,15 +58,20 @@ inline Cfg make_cfg(const opy_tv_cfg_t* c) {
 }
 
 struct TileRef {
-  int b;    // instance
-  int j;    // tile index within the instance
+  int b;  // instance
+  int j;  // tile index within the instance
   int r0, c0;
 };
 
+__device__ inline int xcd_block() {
+  // blocks are dealt round-robin to the 8 XCDs: give each XCD a contiguous range
+  const int g = gridDim.x;
+  return (g % 8 == 0) ? (blockIdx.x % 8) * (g / 8) + blockIdx.x / 8 : blockIdx.x;
+}
+
 __device__ inline TileRef locate(int g, const int32_t* __restrict__ active_inst, const int32_t* __restrict__ act_off,
-                                 int nact, const opy_tv_inst_t* __restrict__ insts, const Cfg& cfg) {
-  // largest a with act_off[a] <= g
-  int lo = 0, hi = nact - 1;
+                                 int nact, const opy_tv_inst_t* __restrict__ insts, int tw, int th) {
+  int lo = 0, hi = nact - 1;  // largest a with act_off[a] <= g
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (act_off[mid] <= g) lo = mid;
@@ -68,81 +81,29 @@ __device__ inline TileRef locate(int g, const int32_t* __restrict__ active_inst,
   t.b = active_inst[lo];
   t.j = g - act_off[lo];
   const int W = insts[t.b].W;
-  const int ntx = (W + cfg.tw - 1) / cfg.tw;
-  const int th = (kTB / cfg.tw) * cfg.rpt;
-  t.r0 = (t.j / ntx) * th;
-  t.c0 = (t.j - (t.j / ntx) * ntx) * cfg.tw;
+  const int ntx = (W + tw - 1) / tw;
+  const int ty = t.j / ntx;
+  t.r0 = ty * th;
+  t.c0 = (t.j - ty * ntx) * tw;
   return t;
 }
 
 template <typename T>
-__device__ inline T step_precond_tau(int r, int c, int H, int W, int k, const Cfg& cfg) {
+__device__ inline T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
+template <typename T>
+__device__ inline T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
+
+template <typename T>
+__device__ inline T precond_tau(int r, int c, int H, int W, int k, const Cfg& cfg) {
   // 1 / column sum of |A| (reference pdhg.py:197-200 via rowwise_lp of the divergence)
   const double s = cfg.w[k] * (((r < H - 1) + (r > 0)) * cfg.ih0 + ((c < W - 1) + (c > 0)) * cfg.ih1);
-  return (T)(s > 2.220446049250313e-16 ? 1.0 / s : s);
+  return (T)(s > kEps ? 1.0 / s : s);
 }
 
 template <typename T>
-__device__ inline T step_precond_sigma(bool inner, int k, double ih, const Cfg& cfg) {
+__device__ inline T precond_sigma(bool inner, int k, double ih, const Cfg& cfg) {
   const double s = inner ? 2.0 * cfg.w[k] * ih : 0.0;
-  return (T)(s > 2.220446049250313e-16 ? 1.0 / s : s);
-}
-
-// A^T y at pixel (r, c), channel k: w_k * (-y0(r,c)/h0 [r<H-1] + y0(r-1,c)/h0 [r>0]
-//                                          -y1(r,c)/h1 [c<W-1] + y1(r,c-1)/h1 [c>0])
-template <typename T, int C>
-__device__ inline void adjoint_at(const T* __restrict__ yb, int64_t pix, int r, int c, int H, int W, const Cfg& cfg,
-                                  T* out) {
-  const T* yp = yb + pix * 2 * C;
-  const T ih0 = (T)cfg.ih0, ih1 = (T)cfg.ih1;
-#pragma unroll
-  for (int k = 0; k < C; ++k) {
-    T d = T(0);
-    if (r < H - 1) d -= yp[k] * ih0;
-    if (r > 0) d += yp[k - (int64_t)2 * C * W] * ih0;
-    if (c < W - 1) d -= yp[C + k] * ih1;
-    if (c > 0) d += yp[C + k - 2 * C] * ih1;
-    out[k] = (T)cfg.w[k] * d;
-  }
-}
-
-// grad of (2*xn - xo) (or of one array when xo == nullptr and two == false) at (r, c): g[0..C) axis 0, g[C..2C) axis 1
-template <typename T, int C>
-__device__ inline void grad_at(const T* __restrict__ xa, const T* __restrict__ xb, int64_t pix, int r, int c, int H,
-                               int W, const Cfg& cfg, T* g) {
-  // value of the combined field at an offset: xb ? 2*xa - xb : xa
-  auto val = [&](int64_t p, int k) -> T { return xb ? T(2) * xa[p * C + k] - xb[p * C + k] : xa[p * C + k]; };
-#pragma unroll
-  for (int k = 0; k < C; ++k) {
-    const T v = val(pix, k);
-    g[k] = (r < H - 1) ? (T)(cfg.w[k] * cfg.ih0) * (val(pix + W, k) - v) : T(0);
-    g[C + k] = (c < W - 1) ? (T)(cfg.w[k] * cfg.ih1) * (val(pix + 1, k) - v) : T(0);
-  }
-}
-
-// y+ at one pixel: projection of y + sigma*grad(2xn - xo) onto the ball of radius tvw
-template <typename T, int C>
-__device__ inline void dual_at(const T* __restrict__ xo, const T* __restrict__ xn, const T* __restrict__ yin, int64_t pix,
-                               int r, int c, int H, int W, T sigma, T tvw, const Cfg& cfg, T* yv) {
-  T g[2 * C];
-  grad_at<T, C>(xn, xo, pix, r, c, H, W, cfg, g);
-  const T* yp = yin + pix * 2 * C;
-  T n2 = T(0);
-#pragma unroll
-  for (int e = 0; e < 2 * C; ++e) {
-    T s = sigma;
-    if (cfg.steps == OPY_TV_STEPS_PRECOND) {
-      const int k = e % C;
-      s = (e < C) ? step_precond_sigma<T>(r < H - 1, k, cfg.ih0, cfg) : step_precond_sigma<T>(c < W - 1, k, cfg.ih1, cfg);
-    }
-    yv[e] = yp[e] + s * g[e];
-    n2 += yv[e] * yv[e];
-  }
-  if (n2 > tvw * tvw) {
-    const T sc = tvw / sqrt(n2);
-#pragma unroll
-    for (int e = 0; e < 2 * C; ++e) yv[e] *= sc;
-  }
+  return (T)(s > kEps ? 1.0 / s : s);
 }
 
 template <typename T>
@@ -157,8 +118,63 @@ __device__ inline T prox_data(T v, T f, double tl, int data) {
   return a * (v + b * f);
 }
 
-// ---- pass 1 ---------------------------------------------------------------
 template <typename T, int C>
+__device__ inline void load_c(const T* __restrict__ p, T* v) {
+#pragma unroll
+  for (int k = 0; k < C; ++k) v[k] = p[k];
+}
+
+// (A^T y)_k at a pixel from y(p) (t=0: y0, t=1: y1), y0 of the pixel above and y1 of the pixel to the left
+template <typename T, int C>
+__device__ inline void adjoint_px(const T* y0, const T* y1, const T* yup0, const T* yleft1, int r, int c, int H, int W,
+                                  const Cfg& cfg, T* out) {
+  const T ih0 = (T)cfg.ih0, ih1 = (T)cfg.ih1;
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    T d = T(0);
+    if (r < H - 1) d -= y0[k] * ih0;
+    if (r > 0) d += yup0[k] * ih0;
+    if (c < W - 1) d -= y1[k] * ih1;
+    if (c > 0) d += yleft1[k] * ih1;
+    out[k] = (T)cfg.w[k] * d;
+  }
+}
+
+// gradient at a pixel from its value, the value below and the value to the right
+template <typename T, int C>
+__device__ inline void grad_px(const T* v, const T* vdown, const T* vright, int r, int c, int H, int W, const Cfg& cfg,
+                               T* g) {
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    g[k] = (r < H - 1) ? (T)(cfg.w[k] * cfg.ih0) * (vdown[k] - v[k]) : T(0);
+    g[C + k] = (c < W - 1) ? (T)(cfg.w[k] * cfg.ih1) * (vright[k] - v[k]) : T(0);
+  }
+}
+
+// y+ = proj_{|.| <= tvw}(y + sigma * grad(xbar)) at one pixel
+template <typename T, int C>
+__device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W, T sigma,
+                                T tvw, const Cfg& cfg, T* yp) {
+  T g[2 * C];
+  grad_px<T, C>(xb, xbd, xbr, r, c, H, W, cfg, g);
+  T n2 = T(0);
+#pragma unroll
+  for (int e = 0; e < 2 * C; ++e) {
+    T s = sigma;
+    if (cfg.steps == OPY_TV_STEPS_PRECOND)
+      s = (e < C) ? precond_sigma<T>(r < H - 1, e, cfg.ih0, cfg) : precond_sigma<T>(c < W - 1, e - C, cfg.ih1, cfg);
+    yp[e] = y[e] + s * g[e];
+    n2 += yp[e] * yp[e];
+  }
+  if (n2 > tvw * tvw) {
+    const T sc = tvw / sqrt(n2);
+#pragma unroll
+    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+  }
+}
+
+// ---- pass 1 ---------------------------------------------------------------
+template <typename T, int C, int RPT>
 __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xin, T* __restrict__ xout,
                                                         const T* __restrict__ y, const T* __restrict__ f,
                                                         const opy_tv_inst_t* __restrict__ insts,
@@ -169,36 +185,62 @@ __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xi
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
-  const int nty = kTB / cfg.tw;
-  const int tx = threadIdx.x % cfg.tw, ty = threadIdx.x / cfg.tw;
-  for (int g = blockIdx.x; g < total; g += gridDim.x) {
-    const TileRef tr = locate(g, active_inst, act_off, nact, insts, cfg);
+  const int tw = cfg.tw;
+  const int th = (kTB / tw) * RPT;
+  const int lane = threadIdx.x & 63;
+  const int tx = threadIdx.x % tw, ty = threadIdx.x / tw;
+  const bool lhalo = (tx == 0) || (lane == 0);
+  for (int g = xcd_block(); g < total; g += gridDim.x) {
+    const TileRef tr = locate(g, active_inst, act_off, nact, insts, tw, th);
     const opy_tv_inst_t I = insts[tr.b];
+    const int H = I.H, W = I.W;
     const double tau_s = st[tr.b].tau;
     const int c = tr.c0 + tx;
-    if (c >= I.W) continue;
+    const bool col = c < W;
+    const int rs = tr.r0 + ty * RPT;
     const T* xb = xin + I.off;
     const T* fb = f + I.off;
     const T* yb = y + 2 * I.off;
     T* ob = xout + I.off;
-    for (int rr = 0; rr < cfg.rpt; ++rr) {
-      const int r = tr.r0 + ty + rr * nty;
-      if (r >= I.H) break;
-      const int64_t pix = (int64_t)r * I.W + c;
-      T d[C];
-      adjoint_at<T, C>(yb, pix, r, c, I.H, I.W, cfg, d);
+    T yup0[C];
 #pragma unroll
-      for (int k = 0; k < C; ++k) {
-        const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? step_precond_tau<T>(r, c, I.H, I.W, k, cfg) : (T)tau_s;
-        const T v = xb[pix * C + k] - tau * d[k];
-        ob[pix * C + k] = prox_data<T>(v, fb[pix * C + k], (double)tau * I.lam, I.data);
+    for (int k = 0; k < C; ++k) yup0[k] = T(0);
+    if (col && rs > 0 && rs < H) load_c<T, C>(yb + ((int64_t)(rs - 1) * W + c) * 2 * C, yup0);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = rs + i;
+      const bool ok = col && r < H;
+      const int64_t pix = (int64_t)r * W + c;
+      T y0[C], y1[C], xv[C], fv[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) y0[k] = y1[k] = xv[k] = fv[k] = T(0);
+      if (ok) {
+        load_c<T, C>(yb + pix * 2 * C, y0);
+        load_c<T, C>(yb + pix * 2 * C + C, y1);
+        load_c<T, C>(xb + pix * C, xv);
+        load_c<T, C>(fb + pix * C, fv);
       }
+      T yl1[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) yl1[k] = shfl_up1(y1[k]);
+      if (lhalo && ok && c > 0) load_c<T, C>(yb + (pix - 1) * 2 * C + C, yl1);
+      if (ok) {
+        T d[C];
+        adjoint_px<T, C>(y0, y1, yup0, yl1, r, c, H, W, cfg, d);
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+          const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(r, c, H, W, k, cfg) : (T)tau_s;
+          ob[pix * C + k] = prox_data<T>(xv[k] - tau * d[k], fv[k], (double)tau * I.lam, I.data);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < C; ++k) yup0[k] = y0[k];
     }
   }
 }
 
 // ---- pass 2 ---------------------------------------------------------------
-template <typename T, int C, bool kAdapt>
+template <typename T, int C, int RPT, bool kAdapt>
 __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, const T* __restrict__ xn,
                                                       const T* __restrict__ yin, T* __restrict__ yout,
                                                       const opy_tv_inst_t* __restrict__ insts,
@@ -211,63 +253,180 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
-  const int nty = kTB / cfg.tw;
-  const int tx = threadIdx.x % cfg.tw, ty = threadIdx.x / cfg.tw;
-  for (int g = blockIdx.x; g < total; g += gridDim.x) {
-    const TileRef tr = locate(g, active_inst, act_off, nact, insts, cfg);
+  const int tw = cfg.tw;
+  const int th = (kTB / tw) * RPT;
+  const int lane = threadIdx.x & 63;
+  const int tx = threadIdx.x % tw, ty = threadIdx.x / tw;
+  const bool lhalo = (tx == 0) || (lane == 0);
+  const bool rhalo = (tx == tw - 1) || (lane == 63);
+  for (int g = xcd_block(); g < total; g += gridDim.x) {
+    const TileRef tr = locate(g, active_inst, act_off, nact, insts, tw, th);
     const opy_tv_inst_t I = insts[tr.b];
+    const int H = I.H, W = I.W;
     const T sigma = (T)st[tr.b].sigma;
     const T tau = (T)st[tr.b].tau;
     const T tvw = (T)I.tvw;
     const int c = tr.c0 + tx;
+    const bool col = c < W;
+    const int rs = tr.r0 + ty * RPT;
     const T* xob = xo + I.off;
     const T* xnb = xn + I.off;
     const T* yib = yin + 2 * I.off;
     T* yob = yout + 2 * I.off;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    if (c < I.W) {
-      for (int rr = 0; rr < cfg.rpt; ++rr) {
-        const int r = tr.r0 + ty + rr * nty;
-        if (r >= I.H) break;
-        const int64_t pix = (int64_t)r * I.W + c;
-        T yv[2 * C];
-        dual_at<T, C>(xob, xnb, yib, pix, r, c, I.H, I.W, sigma, tvw, cfg, yv);
+    // current row (rs) of both primal iterates
+    T xoc[C], xnc[C];
 #pragma unroll
-        for (int e = 0; e < 2 * C; ++e) yob[pix * 2 * C + e] = yv[e];
-        if (kAdapt) {
-          // y+ of the upper and left neighbours (recomputed) for A^T y+ at (r, c)
-          T yu[2 * C], yl[2 * C];
-          if (r > 0) dual_at<T, C>(xob, xnb, yib, pix - I.W, r - 1, c, I.H, I.W, sigma, tvw, cfg, yu);
-          if (c > 0) dual_at<T, C>(xob, xnb, yib, pix - 1, r, c - 1, I.H, I.W, sigma, tvw, cfg, yl);
-          const T ih0 = (T)cfg.ih0, ih1 = (T)cfg.ih1;
-          T at_old[C];
-          adjoint_at<T, C>(yib, pix, r, c, I.H, I.W, cfg, at_old);
-          T gx_old[2 * C], gx_new[2 * C];
-          grad_at<T, C>(xob, nullptr, pix, r, c, I.H, I.W, cfg, gx_old);
-          grad_at<T, C>(xnb, nullptr, pix, r, c, I.H, I.W, cfg, gx_new);
-          const T* yp = yib + pix * 2 * C;
+    for (int k = 0; k < C; ++k) xoc[k] = xnc[k] = T(0);
+    if (col && rs < H) {
+      load_c<T, C>(xob + ((int64_t)rs * W + c) * C, xoc);
+      load_c<T, C>(xnb + ((int64_t)rs * W + c) * C, xnc);
+    }
+    // adaptive: y_k and y+ of the row above the segment (t=0 parts)
+    T ykup0[C], ypup0[C];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int k = 0; k < C; ++k) ykup0[k] = ypup0[k] = T(0);
+    if (kAdapt) {
+      const int ru = rs - 1;
+      const bool oku = col && ru >= 0 && ru < H;
+      T xou[C], xnu[C], yku[2 * C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) xou[k] = xnu[k] = yku[k] = yku[C + k] = T(0);
+      const int64_t pu = (int64_t)ru * W + c;
+      if (oku) {
+        load_c<T, C>(xob + pu * C, xou);
+        load_c<T, C>(xnb + pu * C, xnu);
+        load_c<T, 2 * C>(yib + pu * 2 * C, yku);
+      }
+      T xbu[C], xbc[C], xbur[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xbu[k] = T(2) * xnu[k] - xou[k];
+        xbc[k] = T(2) * xnc[k] - xoc[k];
+        xbur[k] = shfl_down1(xbu[k]);
+      }
+      if (rhalo && oku && c + 1 < W) {
+        T a[C], b[C];
+        load_c<T, C>(xob + (pu + 1) * C, a);
+        load_c<T, C>(xnb + (pu + 1) * C, b);
+#pragma unroll
+        for (int k = 0; k < C; ++k) xbur[k] = T(2) * b[k] - a[k];
+      }
+      T ypu[2 * C];
+      yplus_px<T, C>(xbu, xbc, xbur, yku, ru, c, H, W, sigma, tvw, cfg, ypu);
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        ykup0[k] = yku[k];
+        ypup0[k] = ypu[k];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = rs + i;
+      const bool ok = col && r < H;
+      const int64_t pix = (int64_t)r * W + c;
+      // row below
+      T xod[C], xnd[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) xod[k] = xnd[k] = T(0);
+      if (ok && r + 1 < H) {
+        load_c<T, C>(xob + (pix + W) * C, xod);
+        load_c<T, C>(xnb + (pix + W) * C, xnd);
+      }
+      // right neighbour
+      T xor_[C], xnr[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xor_[k] = shfl_down1(xoc[k]);
+        xnr[k] = shfl_down1(xnc[k]);
+      }
+      if (rhalo && ok && c + 1 < W) {
+        load_c<T, C>(xob + (pix + 1) * C, xor_);
+        load_c<T, C>(xnb + (pix + 1) * C, xnr);
+      }
+      T yk[2 * C];
+#pragma unroll
+      for (int e = 0; e < 2 * C; ++e) yk[e] = T(0);
+      if (ok) load_c<T, 2 * C>(yib + pix * 2 * C, yk);
+      T xbc[C], xbd[C], xbr[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xbc[k] = T(2) * xnc[k] - xoc[k];
+        xbd[k] = T(2) * xnd[k] - xod[k];
+        xbr[k] = T(2) * xnr[k] - xor_[k];
+      }
+      T yp[2 * C];
+      yplus_px<T, C>(xbc, xbd, xbr, yk, r, c, H, W, sigma, tvw, cfg, yp);
+      if (ok) {
+#pragma unroll
+        for (int e = 0; e < 2 * C; ++e) yob[pix * 2 * C + e] = yp[e];
+      }
+      if (kAdapt) {
+        // left neighbour's t=1 parts of y_k and y+
+        T ykl1[C], ypl1[C];
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+          ykl1[k] = shfl_up1(yk[C + k]);
+          ypl1[k] = shfl_up1(yp[C + k]);
+        }
+        if (lhalo && ok && c > 0) {
+          T xol[C], xnl[C], xold[C], xnld[C], ykl[2 * C];
+#pragma unroll
+          for (int k = 0; k < C; ++k) xold[k] = xnld[k] = T(0);
+          load_c<T, C>(xob + (pix - 1) * C, xol);
+          load_c<T, C>(xnb + (pix - 1) * C, xnl);
+          if (r + 1 < H) {
+            load_c<T, C>(xob + (pix - 1 + W) * C, xold);
+            load_c<T, C>(xnb + (pix - 1 + W) * C, xnld);
+          }
+          load_c<T, 2 * C>(yib + (pix - 1) * 2 * C, ykl);
+          T xbl[C], xbld[C];
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            T dn = T(0);
-            if (r < I.H - 1) dn -= yv[k] * ih0;
-            if (r > 0) dn += yu[k] * ih0;
-            if (c < I.W - 1) dn -= yv[C + k] * ih1;
-            if (c > 0) dn += yl[C + k] * ih1;
-            const T at_new = (T)cfg.w[k] * dn;
-            const T q = (xob[pix * C + k] - xnb[pix * C + k]) / tau - at_old[k];
-            const T rp = q + at_new;
+            xbl[k] = T(2) * xnl[k] - xol[k];
+            xbld[k] = T(2) * xnld[k] - xold[k];
+          }
+          T ypl[2 * C];
+          yplus_px<T, C>(xbl, xbld, xbc, ykl, r, c - 1, H, W, sigma, tvw, cfg, ypl);
+#pragma unroll
+          for (int k = 0; k < C; ++k) {
+            ykl1[k] = ykl[C + k];
+            ypl1[k] = ypl[C + k];
+          }
+        }
+        if (ok) {
+          T at_old[C], at_new[C];
+          adjoint_px<T, C>(yk, yk + C, ykup0, ykl1, r, c, H, W, cfg, at_old);
+          adjoint_px<T, C>(yp, yp + C, ypup0, ypl1, r, c, H, W, cfg, at_new);
+          T gxo[2 * C], gxn[2 * C];
+          grad_px<T, C>(xoc, xod, xor_, r, c, H, W, cfg, gxo);
+          grad_px<T, C>(xnc, xnd, xnr, r, c, H, W, cfg, gxn);
+#pragma unroll
+          for (int k = 0; k < C; ++k) {
+            const T q = (xoc[k] - xnc[k]) / tau - at_old[k];
+            const T rp = q + at_new[k];
             s0 += (double)rp * (double)rp;
             s1 += (double)q * (double)q;
           }
 #pragma unroll
           for (int e = 0; e < 2 * C; ++e) {
-            const T q = (yp[e] - yv[e]) / sigma - gx_old[e];
-            const T rd = q + gx_new[e];
-            const T gd = q + T(2) * gx_new[e];
+            const T q = (yk[e] - yp[e]) / sigma - gxo[e];
+            const T rd = q + gxn[e];
+            const T gd = q + T(2) * gxn[e];
             s2 += (double)rd * (double)rd;
             s3 += (double)gd * (double)gd;
           }
         }
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+          ykup0[k] = yk[k];
+          ypup0[k] = yp[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xoc[k] = xod[k];
+        xnc[k] = xnd[k];
       }
     }
     if (kAdapt) {
@@ -333,7 +492,7 @@ __global__ void __launch_bounds__(kTB) tv_adapt_kernel(const opy_tv_inst_t* __re
 
 // ---- primal-dual gap (reference pdhg.py:23-44) ------------------------------
 // per tile partials: [P_data, P_tv, D_lin = sum z f, D_quad = sum z^2, max |y_p|, max |z|] with z = A^T y
-template <typename T, int C>
+template <typename T, int C, int RPT>
 __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, const T* __restrict__ y,
                                                      const T* __restrict__ f, const opy_tv_inst_t* __restrict__ insts,
                                                      const int32_t* __restrict__ active_inst,
@@ -344,40 +503,75 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
-  const int nty = kTB / cfg.tw;
-  const int tx = threadIdx.x % cfg.tw, ty = threadIdx.x / cfg.tw;
-  for (int g = blockIdx.x; g < total; g += gridDim.x) {
-    const TileRef tr = locate(g, active_inst, act_off, nact, insts, cfg);
+  const int tw = cfg.tw;
+  const int th = (kTB / tw) * RPT;
+  const int lane = threadIdx.x & 63;
+  const int tx = threadIdx.x % tw, ty = threadIdx.x / tw;
+  const bool lhalo = (tx == 0) || (lane == 0);
+  const bool rhalo = (tx == tw - 1) || (lane == 63);
+  for (int g = xcd_block(); g < total; g += gridDim.x) {
+    const TileRef tr = locate(g, active_inst, act_off, nact, insts, tw, th);
     const opy_tv_inst_t I = insts[tr.b];
+    const int H = I.H, W = I.W;
     const int c = tr.c0 + tx;
+    const bool col = c < W;
+    const int rs = tr.r0 + ty * RPT;
     const T* xb = x + I.off;
     const T* fb = f + I.off;
     const T* yb = y + 2 * I.off;
     double v[6] = {0, 0, 0, 0, 0, 0};
-    if (c < I.W) {
-      for (int rr = 0; rr < cfg.rpt; ++rr) {
-        const int r = tr.r0 + ty + rr * nty;
-        if (r >= I.H) break;
-        const int64_t pix = (int64_t)r * I.W + c;
+    T xc[C], yup0[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) xc[k] = yup0[k] = T(0);
+    if (col && rs < H) load_c<T, C>(xb + ((int64_t)rs * W + c) * C, xc);
+    if (col && rs > 0 && rs < H) load_c<T, C>(yb + ((int64_t)(rs - 1) * W + c) * 2 * C, yup0);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = rs + i;
+      const bool ok = col && r < H;
+      const int64_t pix = (int64_t)r * W + c;
+      T xd[C], yv[2 * C], fv[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) xd[k] = fv[k] = yv[k] = yv[C + k] = T(0);
+      if (ok && r + 1 < H) load_c<T, C>(xb + (pix + W) * C, xd);
+      if (ok) {
+        load_c<T, 2 * C>(yb + pix * 2 * C, yv);
+        load_c<T, C>(fb + pix * C, fv);
+      }
+      T xr[C], yl1[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xr[k] = shfl_down1(xc[k]);
+        yl1[k] = shfl_up1(yv[C + k]);
+      }
+      if (rhalo && ok && c + 1 < W) load_c<T, C>(xb + (pix + 1) * C, xr);
+      if (lhalo && ok && c > 0) load_c<T, C>(yb + (pix - 1) * 2 * C + C, yl1);
+      if (ok) {
         T gx[2 * C], z[C];
-        grad_at<T, C>(xb, nullptr, pix, r, c, I.H, I.W, cfg, gx);
-        adjoint_at<T, C>(yb, pix, r, c, I.H, I.W, cfg, z);
+        grad_px<T, C>(xc, xd, xr, r, c, H, W, cfg, gx);
+        adjoint_px<T, C>(yv, yv + C, yup0, yl1, r, c, H, W, cfg, z);
         double gn2 = 0.0, yn2 = 0.0;
+#pragma unroll
         for (int e = 0; e < 2 * C; ++e) {
           gn2 += (double)gx[e] * (double)gx[e];
-          const double ye = (double)yb[pix * 2 * C + e];
-          yn2 += ye * ye;
+          yn2 += (double)yv[e] * (double)yv[e];
         }
         v[1] += sqrt(gn2);
         v[4] = fmax(v[4], sqrt(yn2));
+#pragma unroll
         for (int k = 0; k < C; ++k) {
-          const double d = (double)xb[pix * C + k] - (double)fb[pix * C + k];
+          const double d = (double)xc[k] - (double)fv[k];
           v[0] += I.data == OPY_TV_DATA_L1 ? fabs(d) : 0.5 * d * d;
           const double zk = (double)z[k];
-          v[2] += zk * (double)fb[pix * C + k];
+          v[2] += zk * (double)fv[k];
           v[3] += zk * zk;
           v[5] = fmax(v[5], fabs(zk));
         }
+      }
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xc[k] = xd[k];
+        yup0[k] = yv[k];
       }
     }
     for (int j = 0; j < 4; ++j) v[j] = block_sum(v[j], sm);
@@ -413,14 +607,13 @@ __global__ void __launch_bounds__(kTB) tv_check_kernel(const opy_tv_inst_t* __re
   v[5] = block_max(v[5], sm);
   if (threadIdx.x != 0) return;
   opy_tv_check_t& R = chk[b];
-  const double eps = 2.220446049250313e-16;
   const double P = I.lam * v[0] + I.tvw * v[1];
   double D, infeas_d, Dc;
   const double ball = fmax(0.0, v[4] - I.tvw);
   if (I.data == OPY_TV_DATA_L1) {
     D = v[2];
     infeas_d = ball + fmax(0.0, v[5] - I.lam);
-    const double s = v[5] > I.lam ? I.lam / v[5] : 1.0;  // rescaled dual point is feasible
+    const double s = v[5] > I.lam ? I.lam / v[5] : 1.0;  // the rescaled dual point is feasible
     Dc = s * v[2];
   } else {
     D = v[2] - 0.5 * v[3] / I.lam;
@@ -430,11 +623,11 @@ __global__ void __launch_bounds__(kTB) tv_check_kernel(const opy_tv_inst_t* __re
   R.objp = P;
   R.objd = D;
   R.gap = P - D;
-  R.relgap = (P - D) / fmax(eps, fabs(D));
+  R.relgap = (P - D) / fmax(kEps, fabs(D));
   R.infeasp = 0.0;
   R.infeasd = infeas_d;
   R.objd_cert = Dc;
-  R.relgap_cert = (P - Dc) / fmax(eps, fabs(Dc));
+  R.relgap_cert = (P - Dc) / fmax(kEps, fabs(Dc));
   R.iters = (int32_t)(*iter_counter + iters_in_block);
   if (!isfinite(P) || !isfinite(D)) {
     R.status = 2;
@@ -495,11 +688,37 @@ int check_cfg(const opy_tv_cfg_t* c) {
   if (!c) return fail(kInvalid, "tv: null config");
   if (c->C < 1 || c->C > 4) return fail(kInvalid, "tv: C=%d outside [1,4]", c->C);
   if (c->tw < 1 || c->tw > kTB || kTB % c->tw) return fail(kInvalid, "tv: tile width %d must divide %d", c->tw, kTB);
-  if (c->rpt < 1 || c->rpt > 64) return fail(kInvalid, "tv: rows per thread %d outside [1,64]", c->rpt);
+  if (c->rpt != 1 && c->rpt != 2 && c->rpt != 4 && c->rpt != 8 && c->rpt != 16)
+    return fail(kInvalid, "tv: rows per thread %d not in {1,2,4,8,16}", c->rpt);
   if (c->steps != OPY_TV_STEPS_SCALAR && c->steps != OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv: bad step mode");
   if (!(c->h0 > 0) || !(c->h1 > 0)) return fail(kInvalid, "tv: grid spacing must be positive");
   return kOk;
 }
+
+// dispatch a kernel template over C (1..4) and RPT (1,2,4,8,16)
+#define OPY_TV_DISPATCH(KERNEL_EXPR_C_R)                                    \
+  switch (cfg->C * 100 + cfg->rpt) {                                        \
+    case 101: KERNEL_EXPR_C_R(1, 1); break;                                 \
+    case 102: KERNEL_EXPR_C_R(1, 2); break;                                 \
+    case 104: KERNEL_EXPR_C_R(1, 4); break;                                 \
+    case 108: KERNEL_EXPR_C_R(1, 8); break;                                 \
+    case 116: KERNEL_EXPR_C_R(1, 16); break;                                \
+    case 201: KERNEL_EXPR_C_R(2, 1); break;                                 \
+    case 202: KERNEL_EXPR_C_R(2, 2); break;                                 \
+    case 204: KERNEL_EXPR_C_R(2, 4); break;                                 \
+    case 208: KERNEL_EXPR_C_R(2, 8); break;                                 \
+    case 216: KERNEL_EXPR_C_R(2, 16); break;                                \
+    case 301: KERNEL_EXPR_C_R(3, 1); break;                                 \
+    case 302: KERNEL_EXPR_C_R(3, 2); break;                                 \
+    case 304: KERNEL_EXPR_C_R(3, 4); break;                                 \
+    case 308: KERNEL_EXPR_C_R(3, 8); break;                                 \
+    case 316: KERNEL_EXPR_C_R(3, 16); break;                                \
+    case 401: KERNEL_EXPR_C_R(4, 1); break;                                 \
+    case 402: KERNEL_EXPR_C_R(4, 2); break;                                 \
+    case 404: KERNEL_EXPR_C_R(4, 4); break;                                 \
+    case 408: KERNEL_EXPR_C_R(4, 8); break;                                 \
+    default: KERNEL_EXPR_C_R(4, 16); break;                                 \
+  }
 
 template <typename T>
 int primal_impl(const T* xin, T* xout, const T* y, const T* f, const opy_tv_inst_t* insts, const opy_adapt_state_t* st,
@@ -510,25 +729,11 @@ int primal_impl(const T* xin, T* xout, const T* y, const T* f, const opy_tv_inst
   if (xin == xout) return fail(kInvalid, "tv_primal: x_in and x_out must differ");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
-  switch (cfg->C) {
-    case 1: hipLaunchKernelGGL((tv_primal_kernel<T, 1>), dim3(grid), dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c); break;
-    case 2: hipLaunchKernelGGL((tv_primal_kernel<T, 2>), dim3(grid), dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c); break;
-    case 3: hipLaunchKernelGGL((tv_primal_kernel<T, 3>), dim3(grid), dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c); break;
-    default: hipLaunchKernelGGL((tv_primal_kernel<T, 4>), dim3(grid), dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c); break;
-  }
+#define OPY_L(CC, RR) \
+  hipLaunchKernelGGL((tv_primal_kernel<T, CC, RR>), dim3(grid), dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c)
+  OPY_TV_DISPATCH(OPY_L)
+#undef OPY_L
   return check_launch("tv_primal");
-}
-
-template <typename T, bool A>
-void launch_dual(int C, int grid, hipStream_t stm, const T* xo, const T* xn, const T* yi, T* yo,
-                 const opy_tv_inst_t* insts, const opy_adapt_state_t* st, const int32_t* ai, const int32_t* ao,
-                 const int32_t* na, const Cfg& c, double* part) {
-  switch (C) {
-    case 1: hipLaunchKernelGGL((tv_dual_kernel<T, 1, A>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, part); break;
-    case 2: hipLaunchKernelGGL((tv_dual_kernel<T, 2, A>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, part); break;
-    case 3: hipLaunchKernelGGL((tv_dual_kernel<T, 3, A>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, part); break;
-    default: hipLaunchKernelGGL((tv_dual_kernel<T, 4, A>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, part); break;
-  }
 }
 
 template <typename T>
@@ -543,8 +748,17 @@ int dual_impl(const T* xo, const T* xn, const T* yi, T* yo, const opy_tv_inst_t*
   if (adapt && cfg->steps == OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv_dual: adaptive and precond exclude each other");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
-  if (adapt) launch_dual<T, true>(cfg->C, grid, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials);
-  else launch_dual<T, false>(cfg->C, grid, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials);
+  if (adapt) {
+#define OPY_L(CC, RR) \
+  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, true>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
+    OPY_TV_DISPATCH(OPY_L)
+#undef OPY_L
+  } else {
+#define OPY_L(CC, RR) \
+  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, false>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
+    OPY_TV_DISPATCH(OPY_L)
+#undef OPY_L
+  }
   return check_launch("tv_dual");
 }
 
@@ -556,12 +770,10 @@ int gap_impl(const T* x, const T* y, const T* f, const opy_tv_inst_t* insts, con
     return fail(kInvalid, "tv_gap: null pointer or empty grid");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
-  switch (cfg->C) {
-    case 1: hipLaunchKernelGGL((tv_gap_kernel<T, 1>), dim3(grid), dim3(kTB), 0, stm, x, y, f, insts, ai, ao, na, c, partials); break;
-    case 2: hipLaunchKernelGGL((tv_gap_kernel<T, 2>), dim3(grid), dim3(kTB), 0, stm, x, y, f, insts, ai, ao, na, c, partials); break;
-    case 3: hipLaunchKernelGGL((tv_gap_kernel<T, 3>), dim3(grid), dim3(kTB), 0, stm, x, y, f, insts, ai, ao, na, c, partials); break;
-    default: hipLaunchKernelGGL((tv_gap_kernel<T, 4>), dim3(grid), dim3(kTB), 0, stm, x, y, f, insts, ai, ao, na, c, partials); break;
-  }
+#define OPY_L(CC, RR) \
+  hipLaunchKernelGGL((tv_gap_kernel<T, CC, RR>), dim3(grid), dim3(kTB), 0, stm, x, y, f, insts, ai, ao, na, c, partials)
+  OPY_TV_DISPATCH(OPY_L)
+#undef OPY_L
   return check_launch("tv_gap");
 }
 

@@ -31,6 +31,8 @@ class GracefulInterruptHandler(object):
         if self.released:
             return False
         if self._installed:
-            signal.signal(self.sig, self.original_handler)
+            # a handler installed outside Python reads back as None; restore the default then
+            orig = self.original_handler if self.original_handler is not None else signal.SIG_DFL
+            signal.signal(self.sig, orig)
         self.released = True
         return True
