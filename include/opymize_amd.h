@@ -296,9 +296,39 @@ int opy_tv_gap_f64(const double* x, const double* y, const double* f, const opy_
 /* per-instance gap, relative gap and termination test (reference pdhg.py:299-305) */
 int opy_tv_check(const opy_tv_inst_t* insts, opy_tv_check_t* checks, const double* partials, int B, double tol_gap,
                  double tol_infeas, int certify, const int64_t* iter_counter, int iters_in_block, void* stream);
+/* cold restart of the instances with flags[b] != 0: x = y = 0 in the given buffers, status running.
+ * No reference counterpart (the fallback of the learned warm start). */
+int opy_tv_reset_f32(const opy_tv_inst_t* insts, const int32_t* flags, int B, int C, float* x, float* y,
+                     opy_tv_check_t* checks, void* stream);
+int opy_tv_reset_f64(const opy_tv_inst_t* insts, const int32_t* flags, int B, int C, double* x, double* y,
+                     opy_tv_check_t* checks, void* stream);
 /* rebuild the running-instance list (active_inst, act_off: B+1 entries) and advance the iteration counter */
 int opy_tv_compact(const opy_tv_inst_t* insts, const opy_tv_check_t* checks, int B, int32_t* active_inst,
                    int32_t* act_off, int32_t* nact, int64_t* iter_counter, int iters_in_block, void* stream);
+
+/* ---- warm-start network: 3x3 convolutions, 32 -> 32 channels, NHWC float32 ----
+ * No counterpart in the reference (its solvers always start from zero or a
+ * user-supplied continue_at, pdhg.py:236-241); these kernels train and run the
+ * network that predicts continue_at for the batched solver.  Matrix-core
+ * BF16 operands with FP32 accumulation. */
+/* y = act(conv(x, W) + b); wt from opy_conv3x3_prep_weights (9*32*32 bf16); act 0 none, 1 ReLU */
+int opy_conv3x3_fwd_bf16(const float* x, float* y, const void* wt, const float* bias, int N, int H, int W, int act,
+                         int grid, void* stream);
+/* per-block partials (grid x (9*32*32 + 32) floats) of dW[tap][co][ci] and dbias[co] */
+int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,
+                           void* stream);
+/* out[i] = sum_b partials[b*len + i] in block order */
+int opy_sum_partials_f32(const float* partials, int nb, int len, float* out, void* stream);
+/* master weights W[tap][ci][co] (float) -> bf16 [tap][co][ci]; flip=1 gives the backward-data weights */
+int opy_conv3x3_prep_weights(const float* w, void* out, int flip, void* stream);
+/* g *= (y > 0) */
+int opy_relu_bwd_f32(float* g, const float* y, int64_t n, void* stream);
+/* weighted squared error over the first nc <= 3 channels: gradient tensor and per-block FP64 loss partials */
+int opy_mse3_f32(const float* out, const float* tgt, float* grad, int64_t npix, int nc, double w0, double w1,
+                 double w2, double* partials, int grid, void* stream);
+/* Adam update of n FP32 parameters (bias-corrected, step >= 1) */
+int opy_adam_f32(float* p, const float* g, float* m, float* v, int n, double lr, double b1, double b2, double eps,
+                 int step, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,339 @@
+// 3x3 convolutions with 32 input and 32 output channels for the warm-start
+// network: forward, backward-data and backward-weights on the gfx950 matrix
+// cores (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation).
+//
+// Activations are NHWC float32 in HBM (32 channels per pixel, zero padded
+// when a layer has fewer), weights are prepared as BF16 in the layout
+// [tap][out channel][in channel] (tap = 3*dy + dx).  A block of 256 threads
+// computes a 16x16 tile of output pixels for all 32 output channels as an
+// implicit GEMM: M = 16 pixels of one tile row, N = 16 output channels,
+// K = 32 input channels per tap.  The input tile with its one-pixel halo is
+// staged in LDS as BF16.
+//
+// MFMA operand layout of the 16x16x32 BF16 instruction (lane l of 64):
+//   A (16 x 32): row l%16, k = 8*(l/16) .. 8*(l/16)+7
+//   B (32 x 16): col l%16, k = 8*(l/16) .. 8*(l/16)+7
+//   D (16 x 16): col l%16, rows 4*(l/16) .. 4*(l/16)+3
+#include "opy_common.h"
+
+namespace opy {
+namespace {
+
+constexpr int kCh = 32;         // channels
+constexpr int kTile = 16;       // output tile side
+constexpr int kHalo = kTile + 2;
+constexpr int kPixStride = kCh + 8;  // bf16 elements per LDS pixel (padding against bank conflicts)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline __bf16 to_bf16(float v) { return (__bf16)v; }
+
+// ---- forward / backward-data -------------------------------------------------
+// y[n,h,w,co] = act(b[co] + sum_{tap,ci} W[tap][co][ci] * x[n, h+dy-1, w+dx-1, ci])
+// act: 0 none, 1 ReLU.  bias may be NULL.
+__global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const float* __restrict__ x, float* __restrict__ y,
+                                                          const __bf16* __restrict__ wt, const float* __restrict__ bias,
+                                                          int N, int H, int W, int act) {
+  __shared__ __align__(16) __bf16 xs[kHalo * kHalo * kPixStride];
+  __shared__ __align__(16) __bf16 ws[9 * kCh * kPixStride];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 9 * kCh * kCh; e += 256) {
+    const int ci = e % kCh, rest = e / kCh;  // rest = tap*32 + co
+    ws[rest * kPixStride + ci] = wt[e];
+  }
+  const int tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
+  const int ntiles = N * tx * ty;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int n = t / (tx * ty);
+    const int rem = t - n * tx * ty;
+    const int h0 = (rem / tx) * kTile, w0 = (rem % tx) * kTile;
+    __syncthreads();  // previous tile done with xs (and ws staged)
+    const float* xb = x + (int64_t)n * H * W * kCh;
+    for (int e = tid; e < kHalo * kHalo * (kCh / 4); e += 256) {
+      const int q = e % (kCh / 4), p = e / (kCh / 4);
+      const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+        v = *reinterpret_cast<const float4*>(xb + ((int64_t)hh * W + ww) * kCh + 4 * q);
+      __bf16* d = xs + p * kPixStride + 4 * q;
+      d[0] = to_bf16(v.x);
+      d[1] = to_bf16(v.y);
+      d[2] = to_bf16(v.z);
+      d[3] = to_bf16(v.w);
+    }
+    __syncthreads();
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;
+      bf16x8 b[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        b[j] = *reinterpret_cast<const bf16x8*>(ws + (tap * kCh + j * 16 + li) * kPixStride + 8 * lk);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int row = wave * 4 + m;  // tile row of this M block
+        const bf16x8 a =
+            *reinterpret_cast<const bf16x8*>(xs + ((row + dy) * kHalo + li + dx) * kPixStride + 8 * lk);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
+      }
+    }
+    float* yb = y + (int64_t)n * H * W * kCh;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int h = h0 + wave * 4 + m;
+      if (h >= H) continue;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int co = j * 16 + li;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int w = w0 + 4 * lk + r;
+          if (w >= W) continue;
+          float v = acc[m][j][r] + bv;
+          if (act == 1) v = v > 0.f ? v : 0.f;
+          yb[((int64_t)h * W + w) * kCh + co] = v;
+        }
+      }
+    }
+  }
+}
+
+// ---- backward-weights ----------------------------------------------------------
+// dW[tap][co][ci] = sum_{n,h,w} g[n,h,w,co] * x[n, h+dy-1, w+dx-1, ci]   (partials per block)
+// dbias[co] = sum g[..., co]  (partials per block, slot 9*32*32 + co)
+// As a GEMM per tap: D (co x ci) += A (co x pixels) * B (pixels x ci).
+__global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                                                            float* __restrict__ partials, int N, int H, int W) {
+  // channel-major tiles: gsT[co][pixel of the 16x16 tile], xsT[ci][pixel of the 18x18 halo tile]
+  constexpr int kGS = kTile * kTile + 8;
+  constexpr int kXS = kHalo * kHalo + 8;
+  __shared__ __align__(16) __bf16 gsT[kCh * kGS];
+  __shared__ __align__(16) __bf16 xsT[kCh * kXS];
+  __shared__ float bsum[256];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
+  const int ntiles = N * tx * ty;
+  // wave w owns taps w, w+4, w+8 (if < 9); 2 x 2 output blocks (co, ci) per tap
+  f32x4 acc[3][2][2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[a][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bacc = 0.f;  // FP32 bias-gradient partial of channel tid % 32 (the channel this thread stages)
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int n = t / (tx * ty);
+    const int rem = t - n * tx * ty;
+    const int h0 = (rem / tx) * kTile, w0 = (rem % tx) * kTile;
+    __syncthreads();
+    const float* xb = x + (int64_t)n * H * W * kCh;
+    const float* gb = g + (int64_t)n * H * W * kCh;
+    for (int e = tid; e < kHalo * kHalo * kCh; e += 256) {
+      const int c = e % kCh, p = e / kCh;
+      const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+      float v = 0.f;
+      if (hh >= 0 && hh < H && ww >= 0 && ww < W) v = xb[((int64_t)hh * W + ww) * kCh + c];
+      xsT[c * kXS + p] = to_bf16(v);
+    }
+    for (int e = tid; e < kTile * kTile * kCh; e += 256) {
+      const int c = e % kCh, p = e / kCh;
+      const int hh = h0 + p / kTile, ww = w0 + p % kTile;
+      float v = 0.f;
+      if (hh < H && ww < W) v = gb[((int64_t)hh * W + ww) * kCh + c];
+      gsT[c * kGS + p] = to_bf16(v);
+      bacc += v;
+    }
+    __syncthreads();
+    // K loop over the 256 tile pixels in chunks of 32 (two tile rows)
+#pragma unroll
+    for (int ta = 0; ta < 3; ++ta) {
+      const int tap = wave + 4 * ta;
+      if (tap >= 9) break;
+      const int dy = tap / 3, dx = tap % 3;
+#pragma unroll 2
+      for (int kc = 0; kc < 8; ++kc) {
+        // pixels of this chunk: rows 2kc, 2kc+1; lane group lk covers 8 pixels: row 2kc + lk/2, cols 8*(lk%2)..+7
+        const int prow = 2 * kc + (lk >> 1), pcol = 8 * (lk & 1);
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          a[i] = *reinterpret_cast<const bf16x8*>(gsT + (i * 16 + li) * kGS + prow * kTile + pcol);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          // x at halo coordinates (prow + dy, pcol + dx .. +7): not 16-byte aligned in general
+          const __bf16* src = xsT + (j * 16 + li) * kXS + (prow + dy) * kHalo + pcol + dx;
+          bf16x8 v;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = src[q];
+          b[j] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[ta][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[ta][i][j], 0, 0, 0);
+      }
+    }
+  }
+  // write this block's partials: D rows = co (4*(l/16)+r within the 16-block), cols = ci (l%16)
+  float* pb = partials + (int64_t)blockIdx.x * (9 * kCh * kCh + kCh);
+#pragma unroll
+  for (int ta = 0; ta < 3; ++ta) {
+    const int tap = wave + 4 * ta;
+    if (tap >= 9) break;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int co = i * 16 + 4 * lk + r;
+          const int ci = j * 16 + li;
+          pb[(tap * kCh + co) * kCh + ci] = acc[ta][i][j][r];
+        }
+  }
+  bsum[tid] = bacc;
+  __syncthreads();
+  if (tid < kCh) {
+    float sb = 0.f;
+    for (int k = 0; k < 256 / kCh; ++k) sb += bsum[tid + kCh * k];
+    pb[9 * kCh * kCh + tid] = sb;
+  }
+}
+
+// sum of nb partial vectors of length len, in block order (deterministic)
+__global__ void sum_partials_kernel(const float* __restrict__ partials, int nb, int len, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; ++b) s += partials[(int64_t)b * len + i];
+  out[i] = s;
+}
+
+// weights: master W[tap][ci][co] (float) -> bf16 [tap][co][ci] (transpose=0, forward)
+// or the backward-data weights Wb[tap][ci][co] = W[8-tap][ci][co] in the [tap][out=ci][in=co] layout
+__global__ void prep_weights_kernel(const float* __restrict__ w, __bf16* __restrict__ out, int flip) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 9 * kCh * kCh) return;
+  const int tap = e / (kCh * kCh);
+  const int r = e % (kCh * kCh);
+  const int o = r / kCh, i = r % kCh;  // out[tap][o][i]
+  if (!flip) out[e] = to_bf16(w[(tap * kCh + i) * kCh + o]);       // W[tap][ci=i][co=o]
+  else out[e] = to_bf16(w[((8 - tap) * kCh + o) * kCh + i]);        // W[8-tap][ci=o][co=i]
+}
+
+// g = g * (y > 0): ReLU backward through the stored activation
+__global__ void relu_bwd_kernel(float* __restrict__ g, const float* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(y[i] > 0.f)) g[i] = 0.f;
+}
+
+// Adam step on n parameters (FP32 master copy)
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// weighted squared error over the first nc channels of NHWC-32 tensors:
+// loss = sum_{p, c<nc} wc[c]*(out - tgt)^2 / npix, grad = d loss / d out (zero for c >= nc);
+// per-block loss partials in FP64
+__global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ out, const float* __restrict__ tgt,
+                                                  float* __restrict__ grad, int64_t npix, int nc, float w0, float w1,
+                                                  float w2, double* __restrict__ partials) {
+  __shared__ double sm[4];
+  const int64_t n = npix * kCh;
+  const float inv = 1.0f / (float)npix;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % kCh);
+    float gv = 0.f;
+    if (c < nc) {
+      const float wc = c == 0 ? w0 : (c == 1 ? w1 : w2);
+      const float d = out[i] - tgt[i];
+      s += (double)wc * (double)d * (double)d;
+      gv = 2.f * wc * d * inv;
+    }
+    grad[i] = gv;
+  }
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s / (double)npix;
+}
+
+}  // namespace
+}  // namespace opy
+
+using namespace opy;
+
+OPY_API int opy_mse3_f32(const float* out, const float* tgt, float* grad, int64_t npix, int nc, double w0, double w1,
+                         double w2, double* partials, int grid, void* s) {
+  if (!out || !tgt || !grad || !partials || npix < 1 || nc < 1 || nc > 3 || grid < 1)
+    return fail(kInvalid, "mse3: bad arguments");
+  hipLaunchKernelGGL(mse_kernel, dim3(grid), dim3(256), 0, as_stream(s), out, tgt, grad, npix, nc, (float)w0,
+                     (float)w1, (float)w2, partials);
+  return check_launch("mse3");
+}
+
+OPY_API int opy_conv3x3_fwd_bf16(const float* x, float* y, const void* wt, const float* bias, int N, int H, int W,
+                                 int act, int grid, void* s) {
+  if (!x || !y || !wt || N < 1 || H < 1 || W < 1 || grid < 1) return fail(kInvalid, "conv3x3_fwd: bad arguments");
+  if (x == y) return fail(kInvalid, "conv3x3_fwd: in-place convolution is not supported");
+  if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_fwd: input must be 16-byte aligned");
+  hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3(grid), dim3(256), 0, as_stream(s), x, y,
+                     static_cast<const __bf16*>(wt), bias, N, H, W, act);
+  return check_launch("conv3x3_fwd");
+}
+
+OPY_API int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,
+                                   void* s) {
+  if (!x || !g || !partials || N < 1 || H < 1 || W < 1 || grid < 1) return fail(kInvalid, "conv3x3_wgrad: bad arguments");
+  hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3(grid), dim3(256), 0, as_stream(s), x, g, partials, N, H, W);
+  return check_launch("conv3x3_wgrad");
+}
+
+OPY_API int opy_sum_partials_f32(const float* partials, int nb, int len, float* out, void* s) {
+  if (!partials || !out || nb < 1 || len < 1) return fail(kInvalid, "sum_partials: bad arguments");
+  hipLaunchKernelGGL(sum_partials_kernel, dim3((len + 255) / 256), dim3(256), 0, as_stream(s), partials, nb, len, out);
+  return check_launch("sum_partials");
+}
+
+OPY_API int opy_conv3x3_prep_weights(const float* w, void* out, int flip, void* s) {
+  if (!w || !out) return fail(kInvalid, "prep_weights: null pointer");
+  hipLaunchKernelGGL(prep_weights_kernel, dim3((9 * 32 * 32 + 255) / 256), dim3(256), 0, as_stream(s), w,
+                     static_cast<__bf16*>(out), flip);
+  return check_launch("prep_weights");
+}
+
+OPY_API int opy_relu_bwd_f32(float* g, const float* y, int64_t n, void* s) {
+  if (n > 0 && (!g || !y)) return fail(kInvalid, "relu_bwd: null pointer");
+  if (n > 0)
+    hipLaunchKernelGGL(relu_bwd_kernel, dim3(grid_for(n)), dim3(256), 0, as_stream(s), g, y, n);
+  return check_launch("relu_bwd");
+}
+
+OPY_API int opy_adam_f32(float* p, const float* g, float* m, float* v, int n, double lr, double b1, double b2,
+                         double eps, int step, void* s) {
+  if (n < 1 || !p || !g || !m || !v || step < 1) return fail(kInvalid, "adam: bad arguments");
+  const float c1 = (float)(1.0 - pow(b1, step)), c2 = (float)(1.0 - pow(b2, step));
+  hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, as_stream(s), p, g, m, v, n, (float)lr,
+                     (float)b1, (float)b2, (float)eps, c1, c2);
+  return check_launch("adam");
+}

@@ -683,6 +683,21 @@ __global__ void __launch_bounds__(kTB) tv_compact_kernel(const opy_tv_inst_t* __
     }
 }
 
+// Cold restart of flagged instances: zero their x and y (buffer 0) and mark
+// them running again.  One block per instance.
+template <typename T>
+__global__ void __launch_bounds__(kTB) tv_reset_kernel(const opy_tv_inst_t* __restrict__ insts,
+                                                       const int32_t* __restrict__ flags, int C, T* __restrict__ x,
+                                                       T* __restrict__ y, opy_tv_check_t* __restrict__ chk) {
+  const int b = blockIdx.x;
+  if (!flags[b]) return;
+  const opy_tv_inst_t I = insts[b];
+  const int64_t n = (int64_t)I.H * I.W * C;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[I.off + i] = T(0);
+  for (int64_t i = threadIdx.x; i < 2 * n; i += blockDim.x) y[2 * I.off + i] = T(0);
+  if (threadIdx.x == 0) chk[b].status = 0;
+}
+
 // ---- host wrappers ---------------------------------------------------------
 int check_cfg(const opy_tv_cfg_t* c) {
   if (!c) return fail(kInvalid, "tv: null config");
@@ -830,6 +845,22 @@ OPY_API int opy_tv_check(const opy_tv_inst_t* insts, opy_tv_check_t* chk, const 
                      certify, iter_counter, iters_in_block);
   return check_launch("tv_check");
 }
+template <typename T>
+static int reset_impl(const opy_tv_inst_t* insts, const int32_t* flags, int B, int C, T* x, T* y, opy_tv_check_t* chk,
+                      void* s) {
+  if (!insts || !flags || !x || !y || !chk || B < 1 || C < 1 || C > 4) return fail(kInvalid, "tv_reset: bad arguments");
+  hipLaunchKernelGGL((tv_reset_kernel<T>), dim3(B), dim3(kTB), 0, as_stream(s), insts, flags, C, x, y, chk);
+  return check_launch("tv_reset");
+}
+OPY_API int opy_tv_reset_f32(const opy_tv_inst_t* insts, const int32_t* flags, int B, int C, float* x, float* y,
+                             opy_tv_check_t* chk, void* s) {
+  return reset_impl<float>(insts, flags, B, C, x, y, chk, s);
+}
+OPY_API int opy_tv_reset_f64(const opy_tv_inst_t* insts, const int32_t* flags, int B, int C, double* x, double* y,
+                             opy_tv_check_t* chk, void* s) {
+  return reset_impl<double>(insts, flags, B, C, x, y, chk, s);
+}
+
 OPY_API int opy_tv_compact(const opy_tv_inst_t* insts, const opy_tv_check_t* chk, int B, int32_t* active_inst,
                            int32_t* act_off, int32_t* nact, int64_t* iter_counter, int iters_in_block, void* s) {
   if (!insts || !chk || !active_inst || !act_off || !nact || !iter_counter || B < 1)

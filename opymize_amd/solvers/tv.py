@@ -49,6 +49,7 @@ _SIGS = {
     "opy_tv_primal": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_dual": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_gap": [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_reset": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4,
 }
 _PLAIN = {
     "opy_tv_adapt": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
@@ -289,13 +290,39 @@ class TVBatch(object):
         self._compact(0)
         return self.results()
 
+    def _cold_restart(self, flags, steps, step_bound, step_factor):
+        """Reset the flagged instances to x = y = 0 with fresh step state."""
+        fl = torch.as_tensor(np.asarray(flags, dtype=np.int32)).cuda()
+        self._call("opy_tv_reset" + self._sfx(), _hip.ptr(self.insts), _hip.ptr(fl), self.B, self.C,
+                   _hip.ptr(self.x[0]), _hip.ptr(self.y[0]), _hip.ptr(self.checks), _hip.stream())
+        st = self.state.cpu().numpy()
+        F = _hip.ADAPT_FIELD
+        for b in np.flatnonzero(flags):
+            st[b, :] = 0.0
+            if steps == "adaptive":
+                st[b, F["tau"]] = st[b, F["sigma"]] = np.sqrt(step_bound)
+                st[b, F["alpha"]] = 0.5
+            else:
+                st[b, F["sigma"]] = np.sqrt(step_bound / step_factor)
+                st[b, F["tau"]] = step_bound / st[b, F["sigma"]]
+        self.state.copy_(torch.as_tensor(st))
+        self._compact(0)
+
     def solve(self, steps="adaptive", step_bound=0.124, step_factor=1.0, tol=1e-5, tol_infeas=None,
               max_iters=20000, block_iters=50, check_every=1, certify=True, use_graph=True, eps_res=None,
-              callback=None, check_start=False, keep_state=False):
+              callback=None, check_start=False, keep_state=False, fallback=False):
         """Iterate until every instance is certified (|relgap| < tol and
         infeasibility < tol_infeas) or max_iters; the test runs every
-        block_iters*check_every iterations.  Returns self.results()."""
+        block_iters*check_every iterations.  Returns self.results().
+
+        fallback=True (for warm starts): the gap of the start point is
+        recorded, and an instance whose gap has not improved at the first
+        test (or is not finite) restarts cold from x = y = 0."""
         tol_infeas = tol if tol_infeas is None else tol_infeas
+        if fallback:
+            check_start = True
+        self.fallbacks = []
+        gap0 = None
         adapt = steps == "adaptive"
         cfg = self._cfg(steps)
         params = _hip.AdaptParams()
@@ -307,6 +334,13 @@ class TVBatch(object):
         self.cur = 0
         if check_start:
             self._check_step(cfg, tol, tol_infeas, certify, 0)
+            if fallback:
+                r0 = self.results()
+                gap0 = np.array([r["gap"] for r in r0])
+                bad = np.array([r["status"] == 2 for r in r0])
+                if bad.any():
+                    self.fallbacks = list(np.flatnonzero(bad))
+                    self._cold_restart(bad, steps, step_bound, step_factor)
             if int(self.nact.item()) == 0:
                 self.iters_run = 0
                 return self.results()
@@ -346,6 +380,16 @@ class TVBatch(object):
                 else:
                     self._check_step(cfg, tol, tol_infeas, certify, since)
                 since = 0
+                if gap0 is not None:
+                    # first test after a warm start: cold restart where the gap did not improve
+                    r1 = self.results()
+                    flags = np.array([(r["status"] == 2) or (r["status"] == 0 and not r["gap"] < g0)
+                                      for r, g0 in zip(r1, gap0)])
+                    flags &= ~np.isin(np.arange(self.B), self.fallbacks)
+                    if flags.any():
+                        self.fallbacks += list(np.flatnonzero(flags))
+                        self._cold_restart(flags, steps, step_bound, step_factor)
+                    gap0 = None
                 nrun = int(self.nact.item())
                 stop = callback(self, done_iters) if callback is not None else False
                 if nrun == 0 or stop:

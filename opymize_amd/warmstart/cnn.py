@@ -1,0 +1,248 @@
+"""Warm-start network for the batched TV solver.
+
+A 6-layer 3x3 convolutional network (32 channels, ReLU) maps the noisy image
+and its data weight lambda to a starting point (x0, y0) of PDHG:
+  input channels:  f - 0.5, log(lambda)/3        (zero padded to 32)
+  output channels: x0 - f, y0_vertical, y0_horizontal  (y0 projected onto |y| <= tvw)
+All layers run on the hand-written matrix-core kernels of libopyhip.so
+(BF16 operands, FP32 accumulation, FP32 master weights, Adam).  It is
+trained on pairs (f, PDHG solution) that the solver itself produces; the
+solver always continues from the prediction and certifies the result with
+the FP64 gap, falling back to a cold start for an instance whose gap does
+not improve.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from opymize_amd import _hip
+
+CH = 32
+_bound = False
+
+
+def _bind():
+    global _bound
+    if _bound:
+        return
+    lib = _hip.load()
+    P, I, L, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    sigs = {
+        "opy_conv3x3_fwd_bf16": [P, P, P, P, I, I, I, I, I, P],
+        "opy_conv3x3_wgrad_bf16": [P, P, P, I, I, I, I, P],
+        "opy_sum_partials_f32": [P, I, I, P, P],
+        "opy_conv3x3_prep_weights": [P, P, I, P],
+        "opy_relu_bwd_f32": [P, P, L, P],
+        "opy_adam_f32": [P, P, P, P, I, D, D, D, D, I, P],
+        "opy_mse3_f32": [P, P, P, L, I, D, D, D, P, I, P],
+    }
+    for k, a in sigs.items():
+        fn = getattr(lib, k)
+        fn.restype, fn.argtypes = ctypes.c_int, a
+    _bound = True
+
+
+def _call(name, *args):
+    lib = _hip.load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise _hip.HipError("%s failed: %s" % (name, lib.opy_last_error().decode()))
+
+
+class ConvNet(object):
+    """Stack of 3x3 conv layers with 32 -> 32 channels (ReLU between layers)."""
+
+    def __init__(self, layers=6, n_in=2, n_out=3, seed=0):
+        _hip.require_device()
+        _bind()
+        self.layers = layers
+        rng = np.random.default_rng(seed)
+        per = 9 * CH * CH + CH
+        flat = np.zeros(layers * per, dtype=np.float32)
+        for l in range(layers):
+            fan_in = 9 * (n_in if l == 0 else CH)
+            Wm = rng.standard_normal((9, CH, CH)) * np.sqrt(2.0 / fan_in)
+            if l == 0:
+                Wm[:, n_in:, :] = 0.0
+            if l == layers - 1:
+                Wm[:, :, n_out:] = 0.0
+                Wm *= 0.1  # start close to "x0 = f, y0 = 0"
+            flat[l * per:l * per + 9 * CH * CH] = Wm.astype(np.float32).ravel()
+        self.per = per
+        self.params = torch.as_tensor(flat).cuda()
+        self.grads = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+        self.n_in, self.n_out = n_in, n_out
+        self._wf = [torch.empty(9 * CH * CH, dtype=torch.bfloat16, device="cuda") for _ in range(layers)]
+        self._wb = [torch.empty(9 * CH * CH, dtype=torch.bfloat16, device="cuda") for _ in range(layers)]
+        self.grid = 8 * torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+        self._dirty = True
+
+    # -- parameters -------------------------------------------------------
+    def W(self, l):
+        return self.params[l * self.per:l * self.per + 9 * CH * CH]
+
+    def b(self, l):
+        return self.params[l * self.per + 9 * CH * CH:(l + 1) * self.per]
+
+    def _prep(self):
+        if not self._dirty:
+            return
+        s = _hip.stream()
+        for l in range(self.layers):
+            _call("opy_conv3x3_prep_weights", _hip.ptr(self.W(l)), _hip.ptr(self._wf[l]), 0, s)
+            _call("opy_conv3x3_prep_weights", _hip.ptr(self.W(l)), _hip.ptr(self._wb[l]), 1, s)
+        self._dirty = False
+
+    def state_dict(self):
+        return {"params": self.params.cpu().numpy(), "layers": self.layers, "n_in": self.n_in, "n_out": self.n_out}
+
+    def load_state_dict(self, d):
+        self.params.copy_(torch.as_tensor(np.asarray(d["params"], dtype=np.float32)))
+        self._dirty = True
+
+    def save(self, path):
+        np.savez(path, **self.state_dict())
+
+    @classmethod
+    def load(cls, path):
+        d = np.load(path)
+        net = cls(layers=int(d["layers"]), n_in=int(d["n_in"]), n_out=int(d["n_out"]))
+        net.load_state_dict(d)
+        return net
+
+    # -- passes ----------------------------------------------------------------
+    def forward(self, x, keep=False, bufs=None):
+        """x: (N, H, W, 32) float32 device tensor.  Returns the output (and the
+        list of activations when keep=True, needed by backward)."""
+        self._prep()
+        N, H, W, C = x.shape
+        assert C == CH and x.dtype == torch.float32 and x.is_contiguous()
+        s = _hip.stream()
+        acts = [x]
+        cur = x
+        for l in range(self.layers):
+            if keep or bufs is None:
+                out = torch.empty_like(x)
+            else:
+                out = bufs[l % 2]
+            relu = 1 if l < self.layers - 1 else 0
+            _call("opy_conv3x3_fwd_bf16", _hip.ptr(cur), _hip.ptr(out), _hip.ptr(self._wf[l]), _hip.ptr(self.b(l)),
+                  N, H, W, relu, self.grid, s)
+            if keep:
+                acts.append(out)
+            cur = out
+        return (cur, acts) if keep else cur
+
+    def backward(self, acts, gout):
+        """Gradients of all parameters into self.grads given dL/d(output)."""
+        N, H, W, _ = gout.shape
+        s = _hip.stream()
+        nb = min(self.grid, N * ((H + 15) // 16) * ((W + 15) // 16))
+        part = torch.empty(nb * self.per, dtype=torch.float32, device="cuda")
+        g = gout
+        for l in range(self.layers - 1, -1, -1):
+            if l < self.layers - 1:
+                _call("opy_relu_bwd_f32", _hip.ptr(g), _hip.ptr(acts[l + 1]), g.numel(), s)
+            _call("opy_conv3x3_wgrad_bf16", _hip.ptr(acts[l]), _hip.ptr(g), _hip.ptr(part), N, H, W, nb, s)
+            gl = self.grads[l * self.per:(l + 1) * self.per]
+            # partials hold dW as [tap][co][ci]; reorder to the master [tap][ci][co] below
+            tmp = torch.empty(self.per, dtype=torch.float32, device="cuda")
+            _call("opy_sum_partials_f32", _hip.ptr(part), nb, self.per, _hip.ptr(tmp), s)
+            gl[:9 * CH * CH].copy_(tmp[:9 * CH * CH].view(9, CH, CH).transpose(1, 2).reshape(-1))
+            gl[9 * CH * CH:].copy_(tmp[9 * CH * CH:])
+            if l > 0:
+                gn = torch.empty_like(g)
+                _call("opy_conv3x3_fwd_bf16", _hip.ptr(g), _hip.ptr(gn), _hip.ptr(self._wb[l]), None, N, H, W, 0,
+                      self.grid, s)
+                g = gn
+        return self.grads
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.step += 1
+        _call("opy_adam_f32", _hip.ptr(self.params), _hip.ptr(self.grads), _hip.ptr(self.m), _hip.ptr(self.v),
+              self.params.numel(), lr, b1, b2, eps, self.step, _hip.stream())
+        self._dirty = True
+
+    def loss_grad(self, out, tgt, wts=(1.0, 1.0, 1.0)):
+        N, H, W, _ = out.shape
+        grad = torch.empty_like(out)
+        nb = 1024
+        part = torch.zeros(nb, dtype=torch.float64, device="cuda")
+        _call("opy_mse3_f32", _hip.ptr(out), _hip.ptr(tgt), _hip.ptr(grad), N * H * W, self.n_out, wts[0], wts[1],
+              wts[2], _hip.ptr(part), nb, _hip.stream())
+        return part, grad
+
+    def train_step(self, x, tgt, lr=1e-3, wts=(1.0, 1.0, 1.0)):
+        out, acts = self.forward(x, keep=True)
+        part, g = self.loss_grad(out, tgt, wts)
+        self.backward(acts, g)
+        self.adam(lr)
+        return part
+
+
+# -- encoding between images / solutions and network tensors --------------------
+def encode_inputs(images, lams):
+    """list of (H, W) host images (same shape) -> (N, H, W, 32) device tensor."""
+    N = len(images)
+    H, W = images[0].shape
+    x = torch.zeros((N, H, W, CH), dtype=torch.float32, device="cuda")
+    f = torch.as_tensor(np.stack(images).astype(np.float32)).cuda()
+    x[..., 0] = f - 0.5
+    x[..., 1] = torch.as_tensor(np.log(np.asarray(lams, dtype=np.float32)) / 3.0).cuda()[:, None, None]
+    return x
+
+
+def encode_targets(images, xs, ys):
+    """Solutions (x (H*W,), y (H*W*2,)) -> (N, H, W, 32) target tensor (x - f, y_vertical, y_horizontal)."""
+    N = len(images)
+    H, W = images[0].shape
+    t = np.zeros((N, H, W, 3), dtype=np.float32)
+    for i in range(N):
+        t[i, ..., 0] = xs[i].reshape(H, W) - images[i]
+        t[i, ..., 1:3] = ys[i].reshape(H, W, 2)
+    out = torch.zeros((N, H, W, CH), dtype=torch.float32, device="cuda")
+    out[..., :3] = torch.as_tensor(t).cuda()
+    return out
+
+
+class WarmStart(object):
+    """Predicts packed (x0, y0) start vectors for a TVBatch of C=1 instances."""
+
+    def __init__(self, net, tvw=1.0, chunk=16):
+        self.net = net
+        self.tvw = tvw
+        self.chunk = chunk
+
+    def predict_into(self, batch, lams):
+        """Writes the prediction into batch.x[0] / batch.y[0] (the start buffers)."""
+        assert batch.C == 1
+        f_all = batch.f
+        groups = {}
+        for b, shp in enumerate(batch.shapes):
+            groups.setdefault(tuple(shp), []).append(b)
+        bufs = None
+        for (H, W), idx in groups.items():
+            for k in range(0, len(idx), self.chunk):
+                sel = idx[k:k + self.chunk]
+                n = len(sel)
+                inp = torch.zeros((n, H, W, CH), dtype=torch.float32, device="cuda")
+                for j, b in enumerate(sel):
+                    off = batch.offs[b]
+                    inp[j, ..., 0] = f_all[off:off + H * W].view(H, W).float() - 0.5
+                    inp[j, ..., 1] = float(np.log(lams[b]) / 3.0)
+                if bufs is None or bufs[0].shape != inp.shape:
+                    bufs = [torch.empty_like(inp), torch.empty_like(inp)]
+                out = self.net.forward(inp, bufs=bufs)
+                for j, b in enumerate(sel):
+                    off = batch.offs[b]
+                    xv = inp[j, ..., 0] + 0.5 + out[j, ..., 0]
+                    batch.x[0][off:off + H * W].copy_(xv.reshape(-1).to(batch.dtype))
+                    ydst = batch.y[0][2 * off:2 * off + 2 * H * W]
+                    ydst.copy_(out[j, ..., 1:3].reshape(-1).to(batch.dtype))
+                    # feasible dual start: project every pixel's pair onto |y| <= tvw
+                    _hip.call("l1norms_proj", batch.dtype, _hip.ptr(ydst), H * W, 1, 2, float(self.tvw), 0,
+                              _hip.stream())

@@ -1,0 +1,111 @@
+"""Warm-start network kernels against the CPU oracle (BF16 operands rounded
+identically, FP32 accumulation on the GPU vs FP64 in the oracle).
+Tolerance: 2e-3 relative to the output scale (FP32 accumulation order over
+K = 288 products plus BF16 re-rounding of intermediate activations)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cnn
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_close(a, b, tol=2e-3):
+    scale = max(np.abs(b).max(), 1e-6)
+    err = np.abs(a - b).max() / scale
+    assert err < tol, "relative error %g" % err
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 21, 37), (3, 5, 70)])
+def test_conv_forward_and_bias_relu(gpu, shape):
+    from opymize_amd.warmstart.cnn import ConvNet, _call
+    from opymize_amd import _hip
+    N, H, W = shape
+    rng = np.random.default_rng(0)
+    net = ConvNet(layers=1, n_in=32, n_out=32, seed=3)
+    x = rng.standard_normal((N, H, W, 32)).astype(np.float32)
+    Wm = net.W(0).cpu().numpy().reshape(9, 32, 32)
+    b = rng.standard_normal(32).astype(np.float32)
+    net.b(0).copy_(torch.as_tensor(b))
+    net._dirty = True
+    net._prep()
+    xd = torch.as_tensor(x).cuda()
+    for act in (0, 1):
+        y = torch.empty_like(xd)
+        _call("opy_conv3x3_fwd_bf16", _hip.ptr(xd), _hip.ptr(y), _hip.ptr(net._wf[0]), _hip.ptr(net.b(0)), N, H, W,
+              act, net.grid, _hip.stream())
+        ref = cnn.conv_fwd(x, Wm, b, relu=bool(act))
+        rel_close(y.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 19, 33)])
+def test_conv_backward_data_and_weights(gpu, shape):
+    from opymize_amd.warmstart.cnn import ConvNet, _call
+    from opymize_amd import _hip
+    N, H, W = shape
+    rng = np.random.default_rng(1)
+    net = ConvNet(layers=1, n_in=32, n_out=32, seed=4)
+    net._prep()
+    Wm = net.W(0).cpu().numpy().reshape(9, 32, 32)
+    x = rng.standard_normal((N, H, W, 32)).astype(np.float32)
+    g = rng.standard_normal((N, H, W, 32)).astype(np.float32)
+    gd, xd = torch.as_tensor(g).cuda(), torch.as_tensor(x).cuda()
+    dx = torch.empty_like(gd)
+    _call("opy_conv3x3_fwd_bf16", _hip.ptr(gd), _hip.ptr(dx), _hip.ptr(net._wb[0]), None, N, H, W, 0, net.grid,
+          _hip.stream())
+    rel_close(dx.cpu().numpy(), cnn.conv_bwd_data(g, Wm))
+    nb = 7
+    part = torch.empty(nb * net.per, dtype=torch.float32, device="cuda")
+    _call("opy_conv3x3_wgrad_bf16", _hip.ptr(xd), _hip.ptr(gd), _hip.ptr(part), N, H, W, nb, _hip.stream())
+    tot = torch.empty(net.per, dtype=torch.float32, device="cuda")
+    _call("opy_sum_partials_f32", _hip.ptr(part), nb, net.per, _hip.ptr(tot), _hip.stream())
+    t = tot.cpu().numpy()
+    dW_ref, db_ref = cnn.conv_bwd_weights(x, g)
+    rel_close(t[:9 * 1024].reshape(9, 32, 32).transpose(0, 2, 1), dW_ref)
+    rel_close(t[9 * 1024:], db_ref, tol=1e-5)
+
+
+def test_network_loss_and_gradients_match_oracle(gpu):
+    from opymize_amd.warmstart.cnn import ConvNet
+    rng = np.random.default_rng(2)
+    N, H, W = 2, 18, 20
+    net = ConvNet(layers=3, n_in=2, n_out=3, seed=5)
+    x = np.zeros((N, H, W, 32), dtype=np.float32)
+    x[..., :2] = rng.standard_normal((N, H, W, 2))
+    tgt = np.zeros_like(x)
+    tgt[..., :3] = rng.standard_normal((N, H, W, 3))
+    params = []
+    for l in range(3):
+        params.append((net.W(l).cpu().numpy().reshape(9, 32, 32).astype(np.float64),
+                       net.b(l).cpu().numpy().astype(np.float64)))
+    xd, td = torch.as_tensor(x).cuda(), torch.as_tensor(tgt).cuda()
+    out, acts = net.forward(xd, keep=True)
+    part, g = net.loss_grad(out, td, (1.0, 0.5, 0.5))
+    net.backward(acts, g)
+    loss_ref, grads_ref = cnn.net_loss_grad(x, tgt, params, wts=(1.0, 0.5, 0.5), nc=3)
+    assert abs(part.sum().item() - loss_ref) < 2e-3 * abs(loss_ref)
+    G = net.grads.cpu().numpy()
+    for l in range(3):
+        gW = G[l * net.per:l * net.per + 9 * 1024].reshape(9, 32, 32)
+        gb = G[l * net.per + 9 * 1024:(l + 1) * net.per]
+        rel_close(gW, grads_ref[l][0], tol=1e-2)
+        rel_close(gb, grads_ref[l][1], tol=1e-2)
+
+
+def test_training_reduces_loss(gpu):
+    """Fit a fixed smoothing target: loss must drop by 10x within 300 Adam steps."""
+    from opymize_amd.warmstart.cnn import ConvNet
+    rng = np.random.default_rng(3)
+    N, H, W = 8, 32, 32
+    net = ConvNet(layers=3, n_in=2, n_out=3, seed=6)
+    x = np.zeros((N, H, W, 32), dtype=np.float32)
+    x[..., 0] = rng.standard_normal((N, H, W))
+    tgt = np.zeros_like(x)
+    k = np.ones((3, 3)) / 9.0
+    from scipy.signal import correlate
+    for n in range(N):
+        tgt[n, ..., 0] = correlate(x[n, ..., 0], k, mode="same") - x[n, ..., 0]
+    xd, td = torch.as_tensor(x).cuda(), torch.as_tensor(tgt).cuda()
+    losses = [net.train_step(xd, td, lr=3e-3).sum().item() for _ in range(300)]
+    assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
