@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OPY_ABI_VERSION 1
+#define OPY_ABI_VERSION 2
 #define OPY_MAX_DIMS 6
 #define OPY_RED_BLOCKS 1024   /* reductions need work buffers of 3*OPY_RED_BLOCKS doubles */
 #define OPY_QUADEPI_MAXM 16
@@ -83,7 +83,9 @@ enum {
   OPY_FN_QUADEPI_IND = 10,   /* QuadEpiInd infeas           (reference epigraph.py:187-197)         */
   OPY_FN_QUADEPI_SUPP = 11,  /* QuadEpiSupp value/infeas    (reference epigraph.py:154-166)         */
   OPY_FN_HUBER = 12,         /* HuberPerspective            (reference epigraph.py:224-236)         */
-  OPY_FN_TRUNCQUADEPI = 13   /* TruncQuadEpiInd (2 infeas)  (reference epigraph.py:256-270)         */
+  OPY_FN_TRUNCQUADEPI = 13,  /* TruncQuadEpiInd (2 infeas)  (reference epigraph.py:256-270)         */
+  OPY_FN_EPIGRAPH_IND = 14,  /* EpigraphInd infeas          (reference epigraph.py:42-47)           */
+  OPY_FN_EPIGRAPH_SUPP = 15  /* EpigraphSupp value/infeas   (reference epigraph.py:115-126)         */
 };
 
 typedef struct {
@@ -99,6 +101,15 @@ typedef struct {
   double qa;             /* quadratic coefficient a (QuadEpi) or alph (Huber / TruncQuadEpi) */
   const double* qb;      /* linear coefficients b (N x M) or NULL */
   const double* qc;      /* constants c (N) or NULL */
+  /* polyhedral epigraphs, points numbered t = j*nfuns + i, width ndim+1 */
+  const uint8_t* ei;     /* I (nfuns x npoints) */
+  const int32_t* ej;     /* J (nregions x nsub) */
+  const double* ev;      /* v (npoints x ndim) */
+  const double* eb;      /* b (nfuns x npoints) */
+  const double* eq;      /* affine pieces (sum of nfaces) x (ndim+1), EpigraphSupp */
+  const int64_t* eq_off; /* piece offsets per point, nregions*nfuns+1 entries */
+  const double* checks;  /* nregions x (ndim+1) x (ndim+1) feasibility rows */
+  int32_t nfuns, npoints, nsub, epad;
 } opy_fn_params_t;
 
 /* ---- library ---------------------------------------------------------- */
@@ -200,6 +211,14 @@ int opy_l1norms_proj_f64(double* x, int64_t N, int M1, int M2, double lbd, int s
 /* QuadEpiProj (in place; shift may be NULL; lbd = +inf for no truncation): proj.py:278-374, proj.cu:110-219 */
 int opy_quadepi_proj_f32(float* x, int64_t N, int M, double alph, const float* shift, double lbd, void* stream);
 int opy_quadepi_proj_f64(double* x, int64_t N, int M, double alph, const double* shift, double lbd, void* stream);
+/* x (nregions, nfuns, ndim+1) <- projection onto the epigraphs of the pieces
+ * f[i]_j* (active-set QP, at most ndim+1 active constraints, ndim <= 3);
+ * I uint8 (nfuns, npoints), J int32 (nregions, nsub), v (npoints, ndim),
+ * b (nfuns, npoints); EpigraphProj._call_gpu, proj.py:388-470, proj.cu:222-831 */
+int opy_epigraph_proj_f32(float* x, const uint8_t* I, const int32_t* J, const float* v, const float* b, int ndim,
+                          int nfuns, int nregions, int npoints, int nsub, int maxiter, double tol, void* stream);
+int opy_epigraph_proj_f64(double* x, const uint8_t* I, const int32_t* J, const double* v, const double* b, int ndim,
+                          int nfuns, int nregions, int npoints, int nsub, int maxiter, double tol, void* stream);
 
 /* ---- PDHG step kernels (reference opymize/solvers/pdhg.py) ------------- */
 /* take_step: zn = z + dir*step*g; pdhg.py:62-65 */

@@ -36,6 +36,8 @@ FN_QUADEPI_IND = 10
 FN_QUADEPI_SUPP = 11
 FN_HUBER = 12
 FN_TRUNCQUADEPI = 13
+FN_EPIGRAPH_IND = 14
+FN_EPIGRAPH_SUPP = 15
 
 
 class Grid(ctypes.Structure):
@@ -68,7 +70,10 @@ class FnParams(ctypes.Structure):
                 ("c", ctypes.c_void_p), ("cs", ctypes.c_double), ("lbd", ctypes.c_double),
                 ("m1", ctypes.c_int32), ("m2", ctypes.c_int32), ("matrixnorm", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("qa", ctypes.c_double), ("qb", ctypes.c_void_p),
-                ("qc", ctypes.c_void_p)]
+                ("qc", ctypes.c_void_p), ("ei", ctypes.c_void_p), ("ej", ctypes.c_void_p),
+                ("ev", ctypes.c_void_p), ("eb", ctypes.c_void_p), ("eq", ctypes.c_void_p),
+                ("eq_off", ctypes.c_void_p), ("checks", ctypes.c_void_p), ("nfuns", ctypes.c_int32),
+                ("npoints", ctypes.c_int32), ("nsub", ctypes.c_int32), ("epad", ctypes.c_int32)]
 
 
 _P, _I, _L, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -100,6 +105,7 @@ _SIGS = {
     "constrain": [_P, _P, _L, _L, _P, _P, _I, _P],
     "l1norms_proj": [_P, _L, _I, _I, _D, _I, _P],
     "quadepi_proj": [_P, _L, _I, _D, _P, _D, _P],
+    "epigraph_proj": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P],
     "pdhg_take_step": [_P, _P, _L, _I, _P, _P, _P],
     "pdhg_dual_step": [_P, _P, _L, _D, _P, _P, _P, _P],
     "pdhg_residual2": [_P, _P, _P, _D, _P, _D, _D, _P, _L, _P, _P, _P],

@@ -330,6 +330,47 @@ struct FnOp {
         m2 = fmax(m2, 0.5 * p.qa * tn * tn - (double)xi[M]);
         return;
       }
+      case OPY_FN_EPIGRAPH_IND: {  // max(0, <v_l, x[:-1]> - x[-1] - b[fun, l]) over the region's points
+        const int nd = width - 1;
+        const int64_t j = i / p.nfuns, f = i - j * p.nfuns;
+        const int32_t* jr = p.ej + j * p.nsub;
+        double inf = 0.0;
+        for (int k = 0; k < p.nsub; ++k) {
+          const int l = jr[k];
+          if (!p.ei[f * p.npoints + l]) continue;
+          double a = -(double)xi[nd] - p.eb[f * p.npoints + l];
+          for (int d = 0; d < nd; ++d) a += p.ev[(int64_t)l * nd + d] * (double)xi[d];
+          inf = fmax(inf, a);
+        }
+        m1 = fmax(m1, inf);
+        return;
+      }
+      case OPY_FN_EPIGRAPH_SUPP: {  // xt * max_faces <eq, (x[:-1]/xt, 1)>, xt = max(eps, |x[-1]|)
+        const int D = width;
+        const int nd = D - 1;
+        const int64_t j = i / p.nfuns;
+        const double* C = p.checks + j * D * D;
+        double inf = 0.0;
+        for (int r = 0; r < D; ++r) {
+          double c = 0.0;
+          for (int d = 0; d < D; ++d) c += C[r * D + d] * (double)xi[d];
+          inf = fmax(inf, -c);
+        }
+        m1 = fmax(m1, inf);
+        const double xt = fmax(2.220446049250313e-16, fabs((double)xi[nd]));
+        const int64_t f0 = p.eq_off[i], f1 = p.eq_off[i + 1];
+        if (f1 > f0) {
+          double best = -INFINITY;
+          for (int64_t q = f0; q < f1; ++q) {
+            const double* e = p.eq + q * D;
+            double v = e[nd];
+            for (int d = 0; d < nd; ++d) v += e[d] * ((double)xi[d] / xt);
+            best = fmax(best, v);
+          }
+          s += xt * best;
+        }
+        return;
+      }
       default:
         return;
     }
@@ -364,6 +405,20 @@ int fn_eval_impl(int kind, const T* x, int64_t npts, int width, const opy_fn_par
     case OPY_FN_QUADEPI_SUPP:
     case OPY_FN_HUBER:
     case OPY_FN_TRUNCQUADEPI:
+      break;
+    case OPY_FN_EPIGRAPH_IND:
+      if (!prm->ei || !prm->ej || !prm->ev || !prm->eb || prm->nfuns < 1 || prm->npoints < 1 || prm->nsub < 1)
+        return fail(kInvalid, "fn_eval: epigraph indicator needs I, J, v, b and their sizes");
+      if (width < 2 || width > 4) return fail(kInvalid, "fn_eval: epigraph width %d outside [2,4]", width);
+      if (npts % prm->nfuns) return fail(kInvalid, "fn_eval: %lld points not a multiple of %d functions",
+                                         (long long)npts, prm->nfuns);
+      break;
+    case OPY_FN_EPIGRAPH_SUPP:
+      if (!prm->eq || !prm->eq_off || !prm->checks || prm->nfuns < 1)
+        return fail(kInvalid, "fn_eval: epigraph support needs pieces, offsets and checks");
+      if (width < 2 || width > 4) return fail(kInvalid, "fn_eval: epigraph width %d outside [2,4]", width);
+      if (npts % prm->nfuns) return fail(kInvalid, "fn_eval: %lld points not a multiple of %d functions",
+                                         (long long)npts, prm->nfuns);
       break;
     default:
       return fail(kInvalid, "fn_eval: unknown kind %d", kind);

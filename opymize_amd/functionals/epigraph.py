@@ -1,10 +1,120 @@
-"""Epigraph functionals of quadratics (reference: opymize/functionals/epigraph.py:129-274)."""
+"""Epigraph functionals: polyhedral (reference: opymize/functionals/epigraph.py:7-127)
+and of quadratics (reference: opymize/functionals/epigraph.py:129-274)."""
 import numpy as np
+import torch
 
 from opymize_amd import Functional, Variable, _hip
 from opymize_amd._devcache import DeviceConstants
 from opymize_amd.functionals._base import fn_eval, vptr
-from opymize_amd.operators.proj import QuadEpiProj
+from opymize_amd.operators.proj import QuadEpiProj, EpigraphProj, epigraph_Ab
+
+
+def _epigraph_arrays(I, J, v, b):
+    I = np.asarray(I, dtype=bool)
+    J = np.asarray(J)
+    v = np.asarray(v, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    assert I.shape == b.shape and I.shape[1] == v.shape[0]
+    return I, J, v, b
+
+
+def _epi_params(const, I, J, v, b):
+    prm = _hip.FnParams()
+    prm.ei = vptr(const.get_u8("I", I))
+    prm.ej = vptr(const.get("J", J.astype(np.int32), torch.int32))
+    prm.ev = vptr(const.get("v", v, torch.float64))
+    prm.eb = vptr(const.get("b", b, torch.float64))
+    prm.nfuns, prm.npoints = I.shape
+    prm.nsub = J.shape[1]
+    return prm
+
+
+class EpigraphInd(Functional):
+    """F(x) = 0 if <v[l], x[j,i,:-1]> - b[i,l] <= x[j,i,-1] for all j, i and
+    l in J[j] with I[i,l], else infinity; value 0, infeasibility the largest
+    violation (reference: epigraph.py:7-53).  The prox is EpigraphProj."""
+    device_step = True  # prox is independent of the step
+
+    def __init__(self, I, If, J, v, b, conj=None):
+        Functional.__init__(self)
+        I, J, v, b = _epigraph_arrays(I, J, v, b)
+        nfuns, npoints = I.shape
+        nregions, nsub = J.shape
+        ndim = v.shape[1]
+        self.I, self.J, self.v = I, J, v
+        self.x = Variable((nregions, nfuns, ndim + 1))
+        self.conj = EpigraphSupp(I, If, J, v, b, conj=self) if conj is None else conj
+        self.A = self.conj.A
+        self.b = self.conj.b  # right-hand side of the constraint system, as in the reference
+        self._fb = b
+        self._prox = EpigraphProj(I, J, v, b, Ab=(self.A, self.b))
+        self._const = DeviceConstants()
+
+    def _gpu_eval(self, x, out):
+        prm = _epi_params(self._const, self.I, self.J, self.v, self._fb)
+        fn_eval(_hip.FN_EPIGRAPH_IND, x, self.v.shape[1] + 1, out, prm)
+
+    def _gpu_finish(self, v):
+        return 0.0, float(v[1])
+
+    def prox(self, tau):
+        return self._prox
+
+
+EpigraphFct = EpigraphInd
+
+
+class EpigraphSupp(Functional):
+    """F(x) = max sum_ji <y, x[j,i]> over y[j,i] in epi(f[i]_j*), i.e. the
+    perspective xt * f[i]_j(x[:-1]/xt) with xt = |x[-1]| evaluated through the
+    affine pieces of f[i]_j given by the faces If[i][j] (index triples of
+    J[j]); infeasibility max(0, -checks[j] x[j,i]) with checks[j] the inverse
+    transpose of the rows (v[J[j,k]], -1), k <= ndim (reference:
+    epigraph.py:55-127)."""
+
+    def __init__(self, I, If, J, v, b, conj=None):
+        Functional.__init__(self)
+        I, J, v, b = _epigraph_arrays(I, J, v, b)
+        nfuns, npoints = I.shape
+        nregions, nsub = J.shape
+        ndim = v.shape[1]
+        self.I, self.J, self.v, self._fb = I, J, v, b
+        self.x = Variable((nregions, nfuns, ndim + 1))
+        self.A, self.b = epigraph_Ab(I, J, v, b)
+
+        # affine pieces g.x + c through the ndim+1 points of each face
+        self.eqns = []
+        for j in range(nregions):
+            for i in range(nfuns):
+                faces = np.asarray(If[i][j], dtype=np.int64).reshape(-1, ndim + 1)
+                pts = v[J[j]][faces]             # (nfaces, ndim+1, ndim)
+                vals = b[i, J[j]][faces]         # (nfaces, ndim+1)
+                eq = np.zeros((faces.shape[0], ndim + 1))
+                if faces.shape[0]:
+                    eq[:, :-1] = np.linalg.solve(pts[:, 1:] - pts[:, :1], (vals[:, 1:] - vals[:, :1])[..., None])[..., 0]
+                    eq[:, -1] = vals[:, 0] - np.einsum("fd,fd->f", eq[:, :-1], pts[:, 0])
+                self.eqns.append(eq)
+        rows = -np.ones((nregions, ndim + 1, ndim + 1))
+        rows[:, :, :-1] = v[J[:, :ndim + 1]]
+        self.checks = np.linalg.inv(rows).transpose(0, 2, 1)
+        self._eq = np.concatenate(self.eqns).reshape(-1) if self.eqns else np.zeros(0)
+        self._eq_off = np.concatenate([[0], np.cumsum([e.shape[0] for e in self.eqns])]).astype(np.int64)
+        self.conj = EpigraphInd(I, If, J, v, b, conj=self) if conj is None else conj
+        self._const = DeviceConstants()
+
+    def _gpu_eval(self, x, out):
+        prm = _hip.FnParams()
+        prm.eq = vptr(self._const.get("eq", self._eq if self._eq.size else np.zeros(1), torch.float64))
+        prm.eq_off = vptr(self._const.get("eq_off", self._eq_off, torch.int64))
+        prm.checks = vptr(self._const.get("checks", self.checks.reshape(-1), torch.float64))
+        prm.nfuns = self.I.shape[0]
+        fn_eval(_hip.FN_EPIGRAPH_SUPP, x, self.v.shape[1] + 1, out, prm)
+
+    def _gpu_finish(self, v):
+        return float(v[0]), float(v[1])
+
+
+EpigraphSupportFct = EpigraphSupp
 
 
 def quad_dual_coefficients(a, b, c):

@@ -111,3 +111,83 @@ class QuadEpiProj(Operator):
             sh = self._const.get("shift", np.asarray(self.shift).ravel(), x.dtype)
         _hip.call("quadepi_proj", t.dtype, _hip.ptr(t), self.N, self.M, self.alph, _hip.ptr(sh),
                   float(self.lbd), _hip.stream())
+
+
+def epigraph_Ab(I, J, v, b):
+    """Constraint system of the epigraph projection as a block-diagonal CSR
+    matrix A and right-hand side b, one block per (region j, function i) in
+    C order, rows (v[l], -1) and b[i, l] for l = J[j, k] with I[i, l]
+    (reference: proj.py:376-386)."""
+    import scipy.sparse as sp
+    I = np.asarray(I, dtype=bool)
+    J = np.asarray(J)
+    v = np.asarray(v, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nfuns = I.shape[0]
+    nregions, nsub = J.shape
+    ndim = v.shape[1]
+    rows, bs = [], []
+    for j in range(nregions):
+        for i in range(nfuns):
+            ls = [l for l in J[j] if I[i, l]]
+            blk = -np.ones((len(ls), ndim + 1))
+            blk[:, :-1] = v[ls]
+            rows.append(blk)
+            bs.append(b[i, ls])
+    A = sp.block_diag(rows, format="csr") if rows else sp.csr_matrix((0, 0))
+    return A, np.concatenate(bs) if bs else np.zeros(0)
+
+
+class EpigraphProj(Operator):
+    """T(x)[j,i] = projection of x[j,i] onto the epigraph of f[i]_j*, the
+    conjugate of the convex restriction of the piecewise linear f[i] (values
+    b[i,k] at the points v[k], k in I[i]) to the region J[j]:
+
+        minimize 0.5*|y - x[j,i]|^2
+        s.t. <v[l], y[:-1]> - y[-1] <= b[i,l] for l in J[j] with I[i,l]
+
+    (reference: proj.py:388-470, proj.cu:222-831).  Solved per (j, i) on the
+    GPU by an active-set method with at most ndim+1 active constraints and
+    ndim*nsubpoints iterations, ndim <= 3."""
+
+    def __init__(self, I, J, v, b, Ab=None):
+        Operator.__init__(self)
+        I = np.asarray(I, dtype=bool)
+        J = np.asarray(J)
+        v = np.asarray(v, dtype=np.float64)
+        b = np.asarray(b, dtype=np.float64)
+        assert I.shape == b.shape
+        assert I.shape[1] == v.shape[0]
+        nfuns, npoints = I.shape
+        nregions, nsubpoints = J.shape
+        ndim = v.shape[1]
+        if not 1 <= ndim <= 3:
+            raise ValueError("EpigraphProj supports 1 <= ndim <= 3, got %d" % ndim)
+        if J.size and (J.min() < 0 or J.max() >= npoints):
+            raise ValueError("J indexes points outside [0, %d)" % npoints)
+        self.I, self.J, self.v, self.b = I, J, v, b
+        self._Ab = Ab
+        self.x = Variable((nregions, nfuns, ndim + 1))
+        self.y = self.x
+        self.maxiter = ndim * nsubpoints
+        self.tol = 1e-9
+        self._const = DeviceConstants()
+
+    @property
+    def Ab(self):
+        if self._Ab is None:
+            self._Ab = epigraph_Ab(self.I, self.J, self.v, self.b)
+        return self._Ab
+
+    def _call_gpu(self, x, y=None, add=False, jacobian=False):
+        assert not add
+        assert not jacobian
+        t = _inplace_target(x, y)
+        nfuns, npoints = self.I.shape
+        nregions, nsub = self.J.shape
+        dI = self._const.get_u8("I", self.I)
+        dJ = self._const.get("J", self.J.astype(np.int32), torch.int32)
+        dv = self._const.get("v", self.v, t.dtype)
+        db = self._const.get("b", self.b, t.dtype)
+        _hip.call("epigraph_proj", t.dtype, _hip.ptr(t), _hip.ptr(dI), _hip.ptr(dJ), _hip.ptr(dv), _hip.ptr(db),
+                  self.v.shape[1], nfuns, nregions, npoints, nsub, self.maxiter, self.tol, _hip.stream())

@@ -134,3 +134,134 @@ def l1data_conj_prox(z, f, lbd, tau):
     """prox of G*(z) = sum z f + indicator(|z| <= lbd): clip(z - tau f, -lbd, lbd)."""
     N, C = f.shape
     return np.clip(z.reshape(N, C) - _tau_nc(tau, N, C) * f, -lbd[None, :], lbd[None, :]).ravel()
+
+
+def epigraph_constraints(I, J, v, b, j, i):
+    """Rows (v[l], -1) and right-hand sides b[i, l] for l = J[j, k] with I[i, l],
+    in the order of J[j] (reference proj.py:376-386, proj.cu:800-808)."""
+    ls = [int(l) for l in J[j] if I[i, l]]
+    A = -np.ones((len(ls), v.shape[1] + 1))
+    A[:, :-1] = v[ls]
+    return A, np.asarray(b[i, ls], dtype=np.float64)
+
+
+def _proj_polyhedron_bruteforce(x, A, rhs, tol=1e-9):
+    """argmin |y - x| s.t. A y <= rhs by enumerating active sets of at most
+    len(x) linearly independent rows: for each set S, y = x - A_S^T mu with
+    A_S y = rhs_S; a KKT point has mu >= 0 and A y <= rhs."""
+    from itertools import combinations
+    D = x.size
+    if A.shape[0] == 0 or np.all(A @ x <= rhs + tol):
+        return x.copy()
+    best, bestd = None, np.inf
+    for m in range(1, min(D, A.shape[0]) + 1):
+        for S in combinations(range(A.shape[0]), m):
+            N = A[list(S)]
+            G = N @ N.T
+            if np.linalg.matrix_rank(G, tol=1e-10) < m:
+                continue
+            mu = np.linalg.solve(G, N @ x - rhs[list(S)])
+            if np.any(mu < -tol):
+                continue
+            y = x - N.T @ mu
+            if np.all(A @ y <= rhs + tol * (1 + np.abs(rhs))):
+                d = np.sum((y - x) ** 2)
+                if d < bestd:
+                    best, bestd = y, d
+    return best
+
+
+def epigraph_proj(x, I, J, v, b):
+    """Projection of every x[j,i] onto the polyhedral epigraph
+    {y : <v[l], y[:-1]> - y[-1] <= b[i,l], l in J[j], I[i,l]}
+    (reference proj.py:388-470: the exact QP solution its CPU path obtains
+    with MOSEK and its GPU path with an active-set method)."""
+    I = np.asarray(I, dtype=bool)
+    v = np.asarray(v, dtype=np.float64)
+    nfuns = I.shape[0]
+    nregions = J.shape[0]
+    D = v.shape[1] + 1
+    x = np.array(x, dtype=np.float64).reshape(nregions, nfuns, D)
+    y = np.empty_like(x)
+    for j in range(nregions):
+        for i in range(nfuns):
+            A, rhs = epigraph_constraints(I, J, v, b, j, i)
+            y[j, i] = _proj_polyhedron_bruteforce(x[j, i], A, rhs)
+    return y.ravel()
+
+
+def epigraph_ind(x, I, J, v, b):
+    """(0, max(0, largest constraint violation)) (reference epigraph.py:42-47)."""
+    I = np.asarray(I, dtype=bool)
+    nfuns, nregions, D = I.shape[0], J.shape[0], np.asarray(v).shape[1] + 1
+    x = np.asarray(x, dtype=np.float64).reshape(nregions, nfuns, D)
+    inf = 0.0
+    for j in range(nregions):
+        for i in range(nfuns):
+            A, rhs = epigraph_constraints(I, J, np.asarray(v, dtype=np.float64), b, j, i)
+            if A.shape[0]:
+                inf = max(inf, float(np.max(A @ x[j, i] - rhs)))
+    return 0.0, inf
+
+
+def epigraph_pieces(I, If, J, v, b):
+    """Affine pieces (g, c) with g.p + c through the ndim+1 points of every
+    face If[i][j] (indices into J[j]); list over (j, i) in C order
+    (reference epigraph.py:90-103)."""
+    v = np.asarray(v, dtype=np.float64)
+    ndim = v.shape[1]
+    out = []
+    for j in range(J.shape[0]):
+        for i in range(np.asarray(I).shape[0]):
+            eqs = []
+            for face in np.asarray(If[i][j]).reshape(-1, ndim + 1):
+                P = v[J[j][face]]
+                f = np.asarray(b, dtype=np.float64)[i, J[j][face]]
+                # [P 1] (g, c) = f
+                M = np.hstack([P, np.ones((ndim + 1, 1))])
+                eqs.append(np.linalg.solve(M, f))
+            out.append(np.array(eqs).reshape(-1, ndim + 1))
+    return out
+
+
+def epigraph_supp(x, I, If, J, v, b):
+    """(sum_ji xt * max_faces (g.x[:-1]/xt + c), max(0, -checks_j x)) with
+    xt = max(eps, |x[-1]|) and checks_j the inverse transpose of the rows
+    (v[J[j,k]], -1), k <= ndim (reference epigraph.py:104-126)."""
+    v = np.asarray(v, dtype=np.float64)
+    nfuns, nregions, D = np.asarray(I).shape[0], J.shape[0], v.shape[1] + 1
+    x = np.asarray(x, dtype=np.float64).reshape(nregions, nfuns, D)
+    pieces = epigraph_pieces(I, If, J, v, b)
+    rows = -np.ones((nregions, D, D))
+    rows[:, :, :-1] = v[J[:, :D]]
+    checks = np.linalg.inv(rows).transpose(0, 2, 1)
+    val, inf = 0.0, 0.0
+    for j in range(nregions):
+        for i in range(nfuns):
+            xi = x[j, i]
+            inf = max(inf, float(np.max(-(checks[j] @ xi))))
+            xt = max(np.spacing(1), abs(xi[-1]))
+            eq = pieces[j * nfuns + i]
+            if eq.shape[0]:
+                val += xt * float(np.max(eq[:, :-1] @ (xi[:-1] / xt) + eq[:, -1]))
+    return val, max(0.0, inf)
+
+
+def epigraph_supp_lp(x, I, J, v, b):
+    """sum_ji max <y, x[j,i]> over the epigraph polyhedron, by linear
+    programming (scipy.optimize.linprog); the definition of EpigraphSupp
+    (reference epigraph.py:55-69).  Finite where x[j,i,-1] < 0 and
+    x[j,i,:-1]/|x[j,i,-1]| lies in the convex hull of the region's points."""
+    from scipy.optimize import linprog
+    I = np.asarray(I, dtype=bool)
+    v = np.asarray(v, dtype=np.float64)
+    nfuns, nregions, D = I.shape[0], J.shape[0], v.shape[1] + 1
+    x = np.asarray(x, dtype=np.float64).reshape(nregions, nfuns, D)
+    tot = 0.0
+    for j in range(nregions):
+        for i in range(nfuns):
+            A, rhs = epigraph_constraints(I, J, v, b, j, i)
+            r = linprog(-x[j, i], A_ub=A, b_ub=rhs, bounds=[(None, None)] * D, method="highs")
+            assert r.status == 0, r.message
+            tot += -r.fun
+    return tot

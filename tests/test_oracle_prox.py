@@ -149,3 +149,82 @@ def test_l1data_prox_and_moreau():
         assert abs(grid[np.argmin(obj)] - p[i]) < 1e-5
     q = prox.l1data_conj_prox(x / t, f, lbd, 1.0 / t)
     np.testing.assert_allclose(p + t * q, x, atol=1e-12)
+
+
+# ---- polyhedral epigraphs -------------------------------------------------
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_epigraph_proj_reference_construction(ndim):
+    """The reference's tests/epigraph.py:11-59: points above the epigraph are
+    fixed, points pushed out along a facet normal return to the facet."""
+    from tests.epi_cases import reference_proj_case
+    for seed in range(10):
+        I, J, v, b, x_on, x_in, x_out = reference_proj_case(ndim, np.random.default_rng(seed))
+        np.testing.assert_allclose(prox.epigraph_proj(x_in, I, J, v, b), x_in.ravel(), atol=1e-12)
+        np.testing.assert_allclose(prox.epigraph_proj(x_out, I, J, v, b), x_on.ravel(), atol=1e-9)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_epigraph_proj_variational_inequality(ndim):
+    """y = P(x) is feasible and <x - y, z - y> <= 0 for feasible z (the
+    characterisation of the projection onto a closed convex set); checked
+    against an independent solver (scipy SLSQP) as well."""
+    from scipy.optimize import minimize
+    from tests.epi_cases import simplex_case
+    rng = np.random.default_rng(ndim)
+    I, If, J, v, b = simplex_case(ndim, 2, 3, 3, rng, drop=0.3)
+    x = rng.normal(0, 2, (2, 3, ndim + 1))
+    y = prox.epigraph_proj(x, I, J, v, b).reshape(x.shape)
+    for j in range(2):
+        for i in range(3):
+            A, rhs = prox.epigraph_constraints(I, J, v, b, j, i)
+            assert np.all(A @ y[j, i] <= rhs + 1e-9)
+            for _ in range(20):  # random feasible points: lift above the constraints
+                z = rng.normal(0, 3, ndim + 1)
+                z[-1] += max(0.0, np.max(A @ z - rhs)) + rng.random()
+                assert (x[j, i] - y[j, i]) @ (z - y[j, i]) <= 1e-9
+            r = minimize(lambda q: 0.5 * np.sum((q - x[j, i]) ** 2), x[j, i] + 0, jac=lambda q: q - x[j, i],
+                         constraints=[{"type": "ineq", "fun": lambda q: rhs - A @ q, "jac": lambda q: -A}],
+                         method="SLSQP", options={"ftol": 1e-14, "maxiter": 500})
+            np.testing.assert_allclose(y[j, i], r.x, atol=1e-6)
+
+
+def test_epigraph_ind_violation():
+    from tests.epi_cases import simplex_case
+    rng = np.random.default_rng(3)
+    I, If, J, v, b = simplex_case(2, 3, 2, 2, rng)
+    x = rng.normal(0, 1, (3, 2, 3))
+    x[..., -1] = -20.0  # below every epigraph: projections land on the boundary
+    y = prox.epigraph_proj(x, I, J, v, b)
+    assert prox.epigraph_ind(y, I, J, v, b)[1] <= 1e-9
+    y = y.reshape(3, 2, 3)
+    y[1, 0, -1] -= 0.25  # lowering the height by t violates the active constraints by t
+    _, inf = prox.epigraph_ind(y, I, J, v, b)
+    A, rhs = prox.epigraph_constraints(I, J, v, b, 1, 0)
+    assert abs(inf - max(0.0, np.max(A @ y[1, 0] - rhs))) < 1e-12
+    assert inf >= 0.25 - 1e-9
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_epigraph_supp_is_lp_value(ndim):
+    """The faces formula equals max <y, x> over the epigraph polyhedron."""
+    from tests.epi_cases import simplex_case, supp_points
+    rng = np.random.default_rng(10 + ndim)
+    I, If, J, v, b = simplex_case(ndim, 3, 2, 4, rng)
+    x = supp_points(J, v, 2, rng)
+    val, inf = prox.epigraph_supp(x, I, If, J, v, b)
+    assert inf <= 1e-12
+    np.testing.assert_allclose(val, prox.epigraph_supp_lp(x, I, J, v, b), rtol=1e-9, atol=1e-9)
+
+
+def test_epigraph_supp_converges_to_perspective():
+    """The reference's tests/epigraph.py:61-105: sampled quadratics, the error
+    against the perspective of the quadratic falls as the sampling refines."""
+    from tests.epi_cases import support_1d_case
+    errs = []
+    for n in [5, 50, 500]:
+        I, If, J, v, b, x, exact = support_1d_case(n, np.random.default_rng(1))
+        val, inf = prox.epigraph_supp(x, I, If, J, v, b)
+        assert inf <= 1e-12
+        errs.append(abs(val - exact))
+    assert errs[1] < errs[0] and errs[2] < errs[1] and errs[2] < 1e-3
