@@ -32,6 +32,7 @@
 #include "opy_common.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace opy {
 namespace {
@@ -118,10 +119,75 @@ __device__ inline T prox_data(T v, T f, double tl, int data) {
   return a * (v + b * f);
 }
 
-template <typename T, int C>
+// N consecutive values from an address aligned to min(N, 4) * sizeof(T) for
+// even N (per-pixel vectors inside instance blocks whose offsets are
+// multiples of the vector length): one vector load instead of N scalar ones.
+template <typename T, int N>
 __device__ inline void load_c(const T* __restrict__ p, T* v) {
+  if constexpr (sizeof(T) == 4 && N % 4 == 0) {
 #pragma unroll
-  for (int k = 0; k < C; ++k) v[k] = p[k];
+    for (int k = 0; k < N; k += 4) {
+      const float4 q = *reinterpret_cast<const float4*>(p + k);
+      v[k] = q.x;
+      v[k + 1] = q.y;
+      v[k + 2] = q.z;
+      v[k + 3] = q.w;
+    }
+  } else if constexpr (N % 2 == 0) {
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const V2 q = *reinterpret_cast<const V2*>(p + k);
+      v[k] = q.x;
+      v[k + 1] = q.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = p[k];
+  }
+}
+
+template <typename T, int N>
+__device__ inline void store_c(T* __restrict__ p, const T* v) {
+  if constexpr (sizeof(T) == 4 && N % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k += 4) *reinterpret_cast<float4*>(p + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+  } else if constexpr (N % 2 == 0) {
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      V2 q;
+      q.x = v[k];
+      q.y = v[k + 1];
+      *reinterpret_cast<V2*>(p + k) = q;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k] = v[k];
+  }
+}
+
+// sum of four doubles over a 256-thread block, one LDS round; sm is used by
+// tile parity so that consecutive tiles need one barrier each
+__device__ inline void block_sum4(double (&s)[4], double (*sm)[4], int parity) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j] += __shfl_down(s[j], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* buf = sm[parity * (kTB / 64) + wave];
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) buf[j] = s[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double a = 0.0;
+      for (int w = 0; w < kTB / 64; ++w) a += sm[parity * (kTB / 64) + w][j];
+      s[j] = a;
+    }
+  }
 }
 
 // (A^T y)_k at a pixel from y(p) (t=0: y0, t=1: y1), y0 of the pixel above and y1 of the pixel to the left
@@ -151,11 +217,10 @@ __device__ inline void grad_px(const T* v, const T* vdown, const T* vright, int 
   }
 }
 
-// y+ = proj_{|.| <= tvw}(y + sigma * grad(xbar)) at one pixel
+// y+ = proj_{|.| <= tvw}(y + sigma * grad(xbar)) at one pixel; g receives grad(xbar)
 template <typename T, int C>
 __device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W, T sigma,
-                                T tvw, const Cfg& cfg, T* yp) {
-  T g[2 * C];
+                                T tvw, const Cfg& cfg, T* yp, T* g) {
   grad_px<T, C>(xb, xbd, xbr, r, c, H, W, cfg, g);
   T n2 = T(0);
 #pragma unroll
@@ -202,39 +267,54 @@ __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xi
     const T* fb = f + I.off;
     const T* yb = y + 2 * I.off;
     T* ob = xout + I.off;
-    T yup0[C];
+    // every load of the segment is issued before any arithmetic, so that a
+    // wavefront keeps 3*RPT+1 (+RPT halo) loads in flight
+    T yu[C];            // t=0 part of y in the row above the segment
+    T yv[RPT][2 * C];   // y of the own rows
+    T xv[RPT][C], fv[RPT][C];
+    T yl[RPT][C];       // t=1 part of y of the left neighbour (halo lanes)
 #pragma unroll
-    for (int k = 0; k < C; ++k) yup0[k] = T(0);
-    if (col && rs > 0 && rs < H) load_c<T, C>(yb + ((int64_t)(rs - 1) * W + c) * 2 * C, yup0);
+    for (int k = 0; k < C; ++k) yu[k] = T(0);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int k = 0; k < C; ++k) yv[i][k] = yv[i][C + k] = xv[i][k] = fv[i][k] = yl[i][k] = T(0);
+    if (col && rs > 0 && rs < H) load_c<T, C>(yb + ((int64_t)(rs - 1) * W + c) * 2 * C, yu);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int64_t pix = (int64_t)(rs + i) * W + c;
+      if (col && rs + i < H) {
+        load_c<T, 2 * C>(yb + pix * 2 * C, yv[i]);
+        load_c<T, C>(xb + pix * C, xv[i]);
+        load_c<T, C>(fb + pix * C, fv[i]);
+      }
+    }
+    if (lhalo && col && c > 0) {
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+        if (rs + i < H) load_c<T, C>(yb + ((int64_t)(rs + i) * W + c - 1) * 2 * C + C, yl[i]);
+    }
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = rs + i;
       const bool ok = col && r < H;
       const int64_t pix = (int64_t)r * W + c;
-      T y0[C], y1[C], xv[C], fv[C];
-#pragma unroll
-      for (int k = 0; k < C; ++k) y0[k] = y1[k] = xv[k] = fv[k] = T(0);
-      if (ok) {
-        load_c<T, C>(yb + pix * 2 * C, y0);
-        load_c<T, C>(yb + pix * 2 * C + C, y1);
-        load_c<T, C>(xb + pix * C, xv);
-        load_c<T, C>(fb + pix * C, fv);
-      }
       T yl1[C];
 #pragma unroll
-      for (int k = 0; k < C; ++k) yl1[k] = shfl_up1(y1[k]);
-      if (lhalo && ok && c > 0) load_c<T, C>(yb + (pix - 1) * 2 * C + C, yl1);
+      for (int k = 0; k < C; ++k) {
+        const T s = shfl_up1(yv[i][C + k]);
+        yl1[k] = lhalo ? yl[i][k] : s;
+      }
       if (ok) {
-        T d[C];
-        adjoint_px<T, C>(y0, y1, yup0, yl1, r, c, H, W, cfg, d);
+        T d[C], o[C];
+        adjoint_px<T, C>(yv[i], yv[i] + C, i == 0 ? yu : yv[i - 1], yl1, r, c, H, W, cfg, d);
 #pragma unroll
         for (int k = 0; k < C; ++k) {
           const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(r, c, H, W, k, cfg) : (T)tau_s;
-          ob[pix * C + k] = prox_data<T>(xv[k] - tau * d[k], fv[k], (double)tau * I.lam, I.data);
+          o[k] = prox_data<T>(xv[i][k] - tau * d[k], fv[i][k], (double)tau * I.lam, I.data);
         }
+        store_c<T, C>(ob + pix * C, o);
       }
-#pragma unroll
-      for (int k = 0; k < C; ++k) yup0[k] = y0[k];
     }
   }
 }
@@ -249,7 +329,8 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
                                                       const int32_t* __restrict__ act_off,
                                                       const int32_t* __restrict__ nact_p, Cfg cfg,
                                                       double* __restrict__ partials) {
-  __shared__ double sm[kTB / 64];
+  __shared__ double sm[2 * (kTB / 64)][4];
+  int parity = 0;
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
@@ -273,173 +354,158 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
     const T* xnb = xn + I.off;
     const T* yib = yin + 2 * I.off;
     T* yob = yout + 2 * I.off;
-    // current row (rs) of both primal iterates
-    T xoc[C], xnc[C];
+    // Every load of the segment is issued before any arithmetic.  Row index
+    // j of the arrays is image row rs-1+j (row rs-1 only with adaptive steps,
+    // whose residual needs y+ of the row above); the left-neighbour arrays
+    // start at row rs.
+    constexpr int NR = RPT + 2;
+    T ax[NR][C], bx[NR][C];             // x_k, x_k+1 -> later x_k - x_k+1, 2x_k+1 - x_k
+    T yk[RPT + 1][2 * C];               // y_k
+    T axr[RPT + 1][C], bxr[RPT + 1][C];  // right neighbour (right halo lanes), rows rs-1..
+    T axl[RPT + 1][C], bxl[RPT + 1][C];  // left neighbour (left halo lanes, adaptive), rows rs..
+    T ykl[RPT][2 * C];                  // y_k of the left neighbour (left halo lanes, adaptive)
 #pragma unroll
-    for (int k = 0; k < C; ++k) xoc[k] = xnc[k] = T(0);
-    if (col && rs < H) {
-      load_c<T, C>(xob + ((int64_t)rs * W + c) * C, xoc);
-      load_c<T, C>(xnb + ((int64_t)rs * W + c) * C, xnc);
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+      for (int k = 0; k < C; ++k) ax[j][k] = bx[j][k] = T(0);
+#pragma unroll
+    for (int j = 0; j <= RPT; ++j)
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        yk[j][k] = yk[j][C + k] = T(0);
+        axr[j][k] = bxr[j][k] = axl[j][k] = bxl[j][k] = T(0);
+        if (j < RPT) ykl[j][k] = ykl[j][C + k] = T(0);
+      }
+    constexpr int j0 = kAdapt ? 0 : 1;
+#pragma unroll
+    for (int j = j0; j < NR; ++j) {
+      const int r = rs - 1 + j;
+      if (col && r >= 0 && r < H) {
+        const int64_t p = (int64_t)r * W + c;
+        load_c<T, C>(xob + p * C, ax[j]);
+        load_c<T, C>(xnb + p * C, bx[j]);
+        if (j <= RPT) load_c<T, 2 * C>(yib + p * 2 * C, yk[j]);
+      }
     }
-    // adaptive: y_k and y+ of the row above the segment (t=0 parts)
-    T ykup0[C], ypup0[C];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (rhalo && col && c + 1 < W) {
 #pragma unroll
-    for (int k = 0; k < C; ++k) ykup0[k] = ypup0[k] = T(0);
+      for (int j = j0; j <= RPT; ++j) {
+        const int r = rs - 1 + j;
+        if (r >= 0 && r < H) {
+          const int64_t p = (int64_t)r * W + c + 1;
+          load_c<T, C>(xob + p * C, axr[j]);
+          load_c<T, C>(xnb + p * C, bxr[j]);
+        }
+      }
+    }
+    if (kAdapt && lhalo && col && c > 0) {
+#pragma unroll
+      for (int j = 0; j <= RPT; ++j) {
+        const int r = rs + j;
+        if (r < H) {
+          const int64_t p = (int64_t)r * W + c - 1;
+          load_c<T, C>(xob + p * C, axl[j]);
+          load_c<T, C>(xnb + p * C, bxl[j]);
+          if (j < RPT) load_c<T, 2 * C>(yib + p * 2 * C, ykl[j]);
+        }
+      }
+    }
+    // (x_k, x_k+1) -> (xd, xbar) = (x_k - x_k+1, 2 x_k+1 - x_k); right neighbours
+    // from the next lane unless loaded by a halo lane
+#pragma unroll
+    for (int j = j0; j < NR; ++j)
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        const T a = ax[j][k], b = bx[j][k];
+        ax[j][k] = a - b;
+        bx[j][k] = T(2) * b - a;
+      }
+#pragma unroll
+    for (int j = j0; j <= RPT; ++j)
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        const T a = axr[j][k], b = bxr[j][k];
+        const T sd = shfl_down1(ax[j][k]), sb = shfl_down1(bx[j][k]);
+        axr[j][k] = rhalo ? a - b : sd;
+        bxr[j][k] = rhalo ? T(2) * b - a : sb;
+        if (kAdapt) {
+          const T al = axl[j][k], bl = bxl[j][k];
+          axl[j][k] = al - bl;
+          bxl[j][k] = T(2) * bl - al;
+        }
+      }
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const T isig = T(1) / sigma, itau = T(1) / tau;
+    T ypu[C];  // t=0 part of y+ in the row above (adaptive)
+#pragma unroll
+    for (int k = 0; k < C; ++k) ypu[k] = T(0);
     if (kAdapt) {
-      const int ru = rs - 1;
-      const bool oku = col && ru >= 0 && ru < H;
-      T xou[C], xnu[C], yku[2 * C];
+      T yp[2 * C], g[2 * C];
+      yplus_px<T, C>(bx[0], bx[1], bxr[0], yk[0], rs - 1, c, H, W, sigma, tvw, cfg, yp, g);
 #pragma unroll
-      for (int k = 0; k < C; ++k) xou[k] = xnu[k] = yku[k] = yku[C + k] = T(0);
-      const int64_t pu = (int64_t)ru * W + c;
-      if (oku) {
-        load_c<T, C>(xob + pu * C, xou);
-        load_c<T, C>(xnb + pu * C, xnu);
-        load_c<T, 2 * C>(yib + pu * 2 * C, yku);
-      }
-      T xbu[C], xbc[C], xbur[C];
-#pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xbu[k] = T(2) * xnu[k] - xou[k];
-        xbc[k] = T(2) * xnc[k] - xoc[k];
-        xbur[k] = shfl_down1(xbu[k]);
-      }
-      if (rhalo && oku && c + 1 < W) {
-        T a[C], b[C];
-        load_c<T, C>(xob + (pu + 1) * C, a);
-        load_c<T, C>(xnb + (pu + 1) * C, b);
-#pragma unroll
-        for (int k = 0; k < C; ++k) xbur[k] = T(2) * b[k] - a[k];
-      }
-      T ypu[2 * C];
-      yplus_px<T, C>(xbu, xbc, xbur, yku, ru, c, H, W, sigma, tvw, cfg, ypu);
-#pragma unroll
-      for (int k = 0; k < C; ++k) {
-        ykup0[k] = yku[k];
-        ypup0[k] = ypu[k];
-      }
+      for (int k = 0; k < C; ++k) ypu[k] = yp[k];
     }
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = rs + i;
       const bool ok = col && r < H;
       const int64_t pix = (int64_t)r * W + c;
-      // row below
-      T xod[C], xnd[C];
-#pragma unroll
-      for (int k = 0; k < C; ++k) xod[k] = xnd[k] = T(0);
-      if (ok && r + 1 < H) {
-        load_c<T, C>(xob + (pix + W) * C, xod);
-        load_c<T, C>(xnb + (pix + W) * C, xnd);
-      }
-      // right neighbour
-      T xor_[C], xnr[C];
-#pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xor_[k] = shfl_down1(xoc[k]);
-        xnr[k] = shfl_down1(xnc[k]);
-      }
-      if (rhalo && ok && c + 1 < W) {
-        load_c<T, C>(xob + (pix + 1) * C, xor_);
-        load_c<T, C>(xnb + (pix + 1) * C, xnr);
-      }
-      T yk[2 * C];
-#pragma unroll
-      for (int e = 0; e < 2 * C; ++e) yk[e] = T(0);
-      if (ok) load_c<T, 2 * C>(yib + pix * 2 * C, yk);
-      T xbc[C], xbd[C], xbr[C];
-#pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xbc[k] = T(2) * xnc[k] - xoc[k];
-        xbd[k] = T(2) * xnd[k] - xod[k];
-        xbr[k] = T(2) * xnr[k] - xor_[k];
-      }
-      T yp[2 * C];
-      yplus_px<T, C>(xbc, xbd, xbr, yk, r, c, H, W, sigma, tvw, cfg, yp);
-      if (ok) {
-#pragma unroll
-        for (int e = 0; e < 2 * C; ++e) yob[pix * 2 * C + e] = yp[e];
-      }
+      T yp[2 * C], g[2 * C];
+      yplus_px<T, C>(bx[i + 1], bx[i + 2], bxr[i + 1], yk[i + 1], r, c, H, W, sigma, tvw, cfg, yp, g);
+      if (ok) store_c<T, 2 * C>(yob + pix * 2 * C, yp);
       if (kAdapt) {
-        // left neighbour's t=1 parts of y_k and y+
+        // t=1 parts of y_k and y+ of the left neighbour
         T ykl1[C], ypl1[C];
 #pragma unroll
         for (int k = 0; k < C; ++k) {
-          ykl1[k] = shfl_up1(yk[C + k]);
+          ykl1[k] = shfl_up1(yk[i + 1][C + k]);
           ypl1[k] = shfl_up1(yp[C + k]);
         }
         if (lhalo && ok && c > 0) {
-          T xol[C], xnl[C], xold[C], xnld[C], ykl[2 * C];
-#pragma unroll
-          for (int k = 0; k < C; ++k) xold[k] = xnld[k] = T(0);
-          load_c<T, C>(xob + (pix - 1) * C, xol);
-          load_c<T, C>(xnb + (pix - 1) * C, xnl);
-          if (r + 1 < H) {
-            load_c<T, C>(xob + (pix - 1 + W) * C, xold);
-            load_c<T, C>(xnb + (pix - 1 + W) * C, xnld);
-          }
-          load_c<T, 2 * C>(yib + (pix - 1) * 2 * C, ykl);
-          T xbl[C], xbld[C];
+          T ypl[2 * C], gl[2 * C];
+          yplus_px<T, C>(bxl[i], bxl[i + 1], bx[i + 1], ykl[i], r, c - 1, H, W, sigma, tvw, cfg, ypl, gl);
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            xbl[k] = T(2) * xnl[k] - xol[k];
-            xbld[k] = T(2) * xnld[k] - xold[k];
-          }
-          T ypl[2 * C];
-          yplus_px<T, C>(xbl, xbld, xbc, ykl, r, c - 1, H, W, sigma, tvw, cfg, ypl);
-#pragma unroll
-          for (int k = 0; k < C; ++k) {
-            ykl1[k] = ykl[C + k];
+            ykl1[k] = ykl[i][C + k];
             ypl1[k] = ypl[C + k];
           }
         }
         if (ok) {
-          T at_old[C], at_new[C];
-          adjoint_px<T, C>(yk, yk + C, ykup0, ykl1, r, c, H, W, cfg, at_old);
-          adjoint_px<T, C>(yp, yp + C, ypup0, ypl1, r, c, H, W, cfg, at_new);
-          T gxo[2 * C], gxn[2 * C];
-          grad_px<T, C>(xoc, xod, xor_, r, c, H, W, cfg, gxo);
-          grad_px<T, C>(xnc, xnd, xnr, r, c, H, W, cfg, gxn);
+          // primal residual (x_k - x_k+1)/tau - A^T(y_k - y+) and its reference term,
+          // dual residual (y_k - y+)/sigma - A(x_k - x_k+1) and (y_k - y+)/sigma + A xbar
+          T at_old[C], at_new[C], gxd[2 * C];
+          adjoint_px<T, C>(yk[i + 1], yk[i + 1] + C, yk[i], ykl1, r, c, H, W, cfg, at_old);
+          adjoint_px<T, C>(yp, yp + C, ypu, ypl1, r, c, H, W, cfg, at_new);
+          grad_px<T, C>(ax[i + 1], ax[i + 2], axr[i + 1], r, c, H, W, cfg, gxd);
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            const T q = (xoc[k] - xnc[k]) / tau - at_old[k];
+            const T q = ax[i + 1][k] * itau - at_old[k];
             const T rp = q + at_new[k];
-            s0 += (double)rp * (double)rp;
-            s1 += (double)q * (double)q;
+            s[0] += (double)rp * (double)rp;
+            s[1] += (double)q * (double)q;
           }
 #pragma unroll
           for (int e = 0; e < 2 * C; ++e) {
-            const T q = (yk[e] - yp[e]) / sigma - gxo[e];
-            const T rd = q + gxn[e];
-            const T gd = q + T(2) * gxn[e];
-            s2 += (double)rd * (double)rd;
-            s3 += (double)gd * (double)gd;
+            const T q = (yk[i + 1][e] - yp[e]) * isig;
+            const T rd = q - gxd[e];
+            const T gd = q + g[e];
+            s[2] += (double)rd * (double)rd;
+            s[3] += (double)gd * (double)gd;
           }
         }
 #pragma unroll
-        for (int k = 0; k < C; ++k) {
-          ykup0[k] = yk[k];
-          ypup0[k] = yp[k];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xoc[k] = xod[k];
-        xnc[k] = xnd[k];
+        for (int k = 0; k < C; ++k) ypu[k] = yp[k];
       }
     }
     if (kAdapt) {
-      s0 = block_sum(s0, sm);
-      s1 = block_sum(s1, sm);
-      s2 = block_sum(s2, sm);
-      s3 = block_sum(s3, sm);
+      block_sum4(s, sm, parity);
+      parity ^= 1;
       if (threadIdx.x == 0) {
         double* p = partials + 4 * (int64_t)(I.tile0 + tr.j);
-        p[0] = s0;
-        p[1] = s1;
-        p[2] = s2;
-        p[3] = s3;
+        p[0] = s[0];
+        p[1] = s[1];
+        p[2] = s[2];
+        p[3] = s[3];
       }
     }
   }

@@ -99,3 +99,32 @@ def test_batch_warm_start_from_solution(gpu):
     bat2.set_start(x, y)
     res = bat2.solve(steps="adaptive", step_bound=0.124, max_iters=1000, tol=1e-6, check_start=True)
     assert all(r["status"] == 1 for r in res) and all(r["iter"] == 0 for r in res)
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("steps", ["adaptive", "precond"])
+@pytest.mark.parametrize("tile", [(128, 8), (256, 16), (64, 4)])
+@pytest.mark.parametrize("C", [1, 2, 3, 4])
+def test_batch_channels_and_wide_tiles(gpu, precision, steps, tile, C):
+    """Multi-channel batches (per-pixel vectors of C and 2C values) and tiles
+    whose rows span several wavefronts, against the oracle after 30 iterations."""
+    from opymize_amd.solvers.tv import TVBatch
+    if C == 4 and tile[1] == 16:
+        pytest.skip("C=4 with 16 rows per thread exceeds the register budget the tuner allows")
+    shapes = [(33, 150), (9, 300), (70, 41)]
+    rng = np.random.default_rng(C)
+    imgs = [np.stack([rects(H, W, 10 * C + i + k) for k in range(C)], axis=2) for i, (H, W) in enumerate(shapes)]
+    lams = [3.0, 9.0, 1.5]
+    data = ["l2", "l1", "l2"]
+    bat = TVBatch(imgs, lams, data, precision=precision, tile=tile)
+    k = 30
+    bat.solve(steps=steps, step_bound=0.124, max_iters=k, block_iters=10, tol=1e-30)
+    xs, ys = bat.solution()
+    tol = 1e-9 if precision == "double" else 5e-4
+    for b, im in enumerate(imgs):
+        H, W = im.shape[:2]
+        prob = opd.tv_problem(im.reshape(-1, C), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps=steps, step_bound=0.124, term_maxiter=k, granularity=k,
+                                  term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < tol, b
+        assert np.abs(ys[b] - yo).max() < tol, b
