@@ -286,6 +286,9 @@ typedef struct {
   int32_t iters;                 /* iterations run when last checked */
 } opy_tv_check_t;
 
+/* The fused kernels run a persistent grid over the tiles of the running
+ * instances: `grid` > 0 launches that many 256-thread blocks, 0 launches the
+ * number of blocks resident at once on the device for that kernel. */
 /* pass 1: x_out = prox_{tau G}(x_in - tau A^T y) on the tiles of the running instances */
 int opy_tv_primal_f32(const float* x_in, float* x_out, const float* y, const float* f, const opy_tv_inst_t* insts,
                       const opy_adapt_state_t* steps, const int32_t* active_inst, const int32_t* act_off,

@@ -32,7 +32,10 @@
 #include "opy_common.h"
 
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 namespace opy {
 namespace {
@@ -147,23 +150,27 @@ __device__ inline void load_c(const T* __restrict__ p, T* v) {
   }
 }
 
+// Streaming stores of the new iterates (nontemporal: at batch sizes beyond
+// the last-level cache they are not re-read from it), vectorised like load_c.
 template <typename T, int N>
 __device__ inline void store_c(T* __restrict__ p, const T* v) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  typedef T v4 __attribute__((ext_vector_type(4)));
   if constexpr (sizeof(T) == 4 && N % 4 == 0) {
 #pragma unroll
-    for (int k = 0; k < N; k += 4) *reinterpret_cast<float4*>(p + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+    for (int k = 0; k < N; k += 4) {
+      const v4 q = {v[k], v[k + 1], v[k + 2], v[k + 3]};
+      __builtin_nontemporal_store(q, reinterpret_cast<v4*>(p + k));
+    }
   } else if constexpr (N % 2 == 0) {
-    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
 #pragma unroll
     for (int k = 0; k < N; k += 2) {
-      V2 q;
-      q.x = v[k];
-      q.y = v[k + 1];
-      *reinterpret_cast<V2*>(p + k) = q;
+      const v2 q = {v[k], v[k + 1]};
+      __builtin_nontemporal_store(q, reinterpret_cast<v2*>(p + k));
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < N; ++k) p[k] = v[k];
+    for (int k = 0; k < N; ++k) __builtin_nontemporal_store(v[k], p + k);
   }
 }
 
@@ -765,6 +772,31 @@ __global__ void __launch_bounds__(kTB) tv_reset_kernel(const opy_tv_inst_t* __re
 }
 
 // ---- host wrappers ---------------------------------------------------------
+#define KF(...) reinterpret_cast<const void*>(&__VA_ARGS__)
+
+// Blocks of `kernel` resident at once on the current device (occupancy x
+// CUs), cached per kernel and device: the persistent loops need no more.
+int resident_blocks(const void* kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(kernel, dev);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTB, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (dev < 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  const int g = per_cu * cus;
+  cache[key] = g;
+  return g;
+}
+
+// grid > 0: as given; grid == 0: the resident block count of the kernel
+inline int launch_grid(const void* kernel, int grid) { return grid > 0 ? grid : resident_blocks(kernel); }
+
 int check_cfg(const opy_tv_cfg_t* c) {
   if (!c) return fail(kInvalid, "tv: null config");
   if (c->C < 1 || c->C > 4) return fail(kInvalid, "tv: C=%d outside [1,4]", c->C);
@@ -805,13 +837,14 @@ template <typename T>
 int primal_impl(const T* xin, T* xout, const T* y, const T* f, const opy_tv_inst_t* insts, const opy_adapt_state_t* st,
                 const int32_t* ai, const int32_t* ao, const int32_t* na, const opy_tv_cfg_t* cfg, int grid, void* s) {
   if (int rc = check_cfg(cfg)) return rc;
-  if (!xin || !xout || !y || !f || !insts || !st || !ai || !ao || !na || grid < 1)
-    return fail(kInvalid, "tv_primal: null pointer or empty grid");
+  if (!xin || !xout || !y || !f || !insts || !st || !ai || !ao || !na || grid < 0)
+    return fail(kInvalid, "tv_primal: null pointer or negative grid");
   if (xin == xout) return fail(kInvalid, "tv_primal: x_in and x_out must differ");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
 #define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_primal_kernel<T, CC, RR>), dim3(grid), dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c)
+  hipLaunchKernelGGL((tv_primal_kernel<T, CC, RR>), dim3(launch_grid(KF(tv_primal_kernel<T, CC, RR>), grid)), dim3(kTB), 0, \
+                     stm, xin, xout, y, f, insts, st, ai, ao, na, c)
   OPY_TV_DISPATCH(OPY_L)
 #undef OPY_L
   return check_launch("tv_primal");
@@ -822,8 +855,8 @@ int dual_impl(const T* xo, const T* xn, const T* yi, T* yo, const opy_tv_inst_t*
               const int32_t* ai, const int32_t* ao, const int32_t* na, const opy_tv_cfg_t* cfg, int adapt,
               double* partials, int grid, void* s) {
   if (int rc = check_cfg(cfg)) return rc;
-  if (!xo || !xn || !yi || !yo || !insts || !st || !ai || !ao || !na || grid < 1)
-    return fail(kInvalid, "tv_dual: null pointer or empty grid");
+  if (!xo || !xn || !yi || !yo || !insts || !st || !ai || !ao || !na || grid < 0)
+    return fail(kInvalid, "tv_dual: null pointer or negative grid");
   if (yi == yo || xo == xn) return fail(kInvalid, "tv_dual: ping-pong buffers must differ");
   if (adapt && !partials) return fail(kInvalid, "tv_dual: adaptive steps need a partials buffer");
   if (adapt && cfg->steps == OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv_dual: adaptive and precond exclude each other");
@@ -831,12 +864,14 @@ int dual_impl(const T* xo, const T* xn, const T* yi, T* yo, const opy_tv_inst_t*
   hipStream_t stm = as_stream(s);
   if (adapt) {
 #define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, true>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
+  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, true>), dim3(launch_grid(KF(tv_dual_kernel<T, CC, RR, true>), grid)), \
+                     dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
     OPY_TV_DISPATCH(OPY_L)
 #undef OPY_L
   } else {
 #define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, false>), dim3(grid), dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
+  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, false>), dim3(launch_grid(KF(tv_dual_kernel<T, CC, RR, false>), grid)), \
+                     dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
     OPY_TV_DISPATCH(OPY_L)
 #undef OPY_L
   }
@@ -847,12 +882,13 @@ template <typename T>
 int gap_impl(const T* x, const T* y, const T* f, const opy_tv_inst_t* insts, const int32_t* ai, const int32_t* ao,
              const int32_t* na, const opy_tv_cfg_t* cfg, double* partials, int grid, void* s) {
   if (int rc = check_cfg(cfg)) return rc;
-  if (!x || !y || !f || !insts || !ai || !ao || !na || !partials || grid < 1)
-    return fail(kInvalid, "tv_gap: null pointer or empty grid");
+  if (!x || !y || !f || !insts || !ai || !ao || !na || !partials || grid < 0)
+    return fail(kInvalid, "tv_gap: null pointer or negative grid");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
 #define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_gap_kernel<T, CC, RR>), dim3(grid), dim3(kTB), 0, stm, x, y, f, insts, ai, ao, na, c, partials)
+  hipLaunchKernelGGL((tv_gap_kernel<T, CC, RR>), dim3(launch_grid(KF(tv_gap_kernel<T, CC, RR>), grid)), dim3(kTB), 0, \
+                     stm, x, y, f, insts, ai, ao, na, c, partials)
   OPY_TV_DISPATCH(OPY_L)
 #undef OPY_L
   return check_launch("tv_gap");

@@ -80,10 +80,6 @@ def exported_names():
     return [k + s for k in _SIGS for s in ("_f32", "_f64")] + list(_PLAIN)
 
 
-def _cu_count():
-    return torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
-
-
 def tile_height(tw, rpt):
     return (256 // tw) * rpt
 
@@ -162,8 +158,8 @@ class TVBatch(object):
         self.act_off = torch.zeros(self.B + 1, dtype=torch.int32, device="cuda")
         self.nact = torch.zeros(1, dtype=torch.int32, device="cuda")
         self.iter_counter = torch.zeros(1, dtype=torch.int64, device="cuda")
-        g = self._grid_override or 8 * _cu_count()
-        self.grid = int(max(1, min(g, t0)))
+        # 0: the library sizes each persistent launch to the blocks resident at once
+        self.grid = int(max(1, min(self._grid_override, t0))) if self._grid_override else 0
         self.graph = None
 
     def _cfg(self, steps):
