@@ -1,18 +1,23 @@
 """Throughput of the PDHG hot path on one MI355X.
 
 Metric: solves/sec to a relative primal-dual gap of 1e-5, certified in FP64,
-for the batched (and, when a trained warm-start network is available,
-warm-started) fused PDHG on the learned-warm-start workload: TV-L2 denoising
-of 1024x1024 images from the rectangle generator, lambda log-uniform in
-[1, 20], 256 instances per batch.  A step = solving one resident batch from
-its start point until every instance is certified or the iteration cap.
+for the batched fused PDHG on the learned-warm-start workload: TV-L2
+denoising of 1024x1024 images from the rectangle generator, lambda
+log-uniform in [1, 20], 256 instances per batch.  A step = solving one
+resident batch from its start point (cold, or the network's prediction with
+--warm) until every instance is certified or the iteration cap.
 
-Printed next to it: the plain single-instance GPU PDHG (the drop-in generic
-kernel chain of PDHG.solve, cold start, same step rule) on a few instances of
-the same batch; the roofline of the dominant kernel (HIP events on its
-stream); and the CPU oracle timed on the host (bounded sample).
+Printed next to it:
+  * the plain single-instance GPU PDHG (the drop-in generic kernel chain of
+    PDHG.solve, cold start, same step rule and cap) on instances spread over
+    the lambda range of the same batch;
+  * the roofline of the dominant kernel (the adaptive dual pass): algorithmic
+    bytes of a full-batch launch over the average duration of eager
+    full-batch launches, timed with HIP events on the launch stream;
+  * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--warm]
+       python bench.py --roofline-only   (only the kernel-timing phase; for counter runs)
 """
 import argparse
 import json
@@ -26,6 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+DUAL_BYTES_PER_PX = 24.0    # adaptive dual pass, C=1 FP32: reads x_k, x_k+1 (4+4 B), y_k (8 B); writes y+ (8 B)
+PRIMAL_BYTES_PER_PX = 20.0  # primal pass: reads x_k, f (4+4 B), y_k (8 B); writes x_k+1 (4 B)
 
 
 def log(*a):
@@ -46,17 +53,19 @@ def build_batch(args):
     return bat, specs
 
 
-def time_dominant_kernel(bat, reps=10):
-    """Average duration of the fused dual pass over the full batch (all
-    instances running), timed with HIP events on the launch stream."""
+def roofline_phase(bat, reps):
+    """Eager full-batch launches of the two passes (every instance running),
+    timed with HIP events on the stream they are launched on; one launch of
+    the library's copy kernel over a buffer of known size for counter scaling."""
     import ctypes
     import torch
     from opymize_amd import _hip
     steps = "adaptive"
     cfg = bat._cfg(steps)
     bat._reset_state(steps, 0.124, 1.0, 1e-5, 1e-5, True)
-    P, s, sfx = _hip.ptr, _hip.stream(), bat._sfx()
+    P, sfx = _hip.ptr, bat._sfx()
     stream = torch.cuda.current_stream()
+    s = _hip.stream()
 
     def launch(name):
         if name == "dual":
@@ -78,18 +87,44 @@ def time_dominant_kernel(bat, reps=10):
         e1.record(stream)
         torch.cuda.synchronize()
         out[name] = e0.elapsed_time(e1) / reps
+    # copy of a known size (2^28 floats = 1 GiB read + 1 GiB written): the counter scale
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    _hip.call("copy", a.dtype, P(a), P(b), a.numel(), s)
+    torch.cuda.synchronize()
+    del a, b
     return out
 
 
+def measured_traffic(npx, tile):
+    """HBM bytes per full-batch dual launch from the committed counter runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of
+    known size), when they were taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "tv_dual_kernel_tvl2_256x1024_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        m = json.load(fh)
+    if m["workload"]["pixels"] != npx or list(m["workload"]["tile"]) != list(tile):
+        return None
+    kib = 1024.0
+    fetch = m["fetch_kib"] * (m["copy_known_bytes_each_way"] / (m["copy_fetch_kib"] * kib))
+    write = m["write_kib"] * (m["copy_known_bytes_each_way"] / (m["copy_write_kib"] * kib))
+    return (fetch + write) * kib
+
+
 def plain_single_instance(bat, specs, n, max_iters):
-    """Drop-in generic PDHG (per-operator kernels, graph-captured) on n instances."""
+    """Drop-in generic PDHG (per-operator kernels, graph-captured), cold start,
+    on n instances spread evenly over the lambda order of the batch."""
     import torch
     from opymize_amd.functionals import SSD, L1Norms
     from opymize_amd.linear import GradientOp
     from opymize_amd.solvers import PDHG
     f_all = bat.unpack(bat.f, "x")
+    order = np.argsort([s["lam"] for s in specs])
+    pick = [int(order[int(round(q))]) for q in np.linspace(0, len(order) - 1, n)]
     solved, iters, t_tot = 0, [], 0.0
-    for b in range(n):
+    for b in pick:
         H, W = bat.shapes[b]
         f = f_all[b].reshape(-1, 1)
         G = SSD(f, vol=np.array([specs[b]["lam"]]))
@@ -102,14 +137,14 @@ def plain_single_instance(bat, specs, n, max_iters):
                        granularity=50, term_pd_gap=1e-5, fused=False)
         torch.cuda.synchronize()
         t_tot += time.perf_counter() - t
-        ok = abs(info["relgap"]) < 1e-5
-        solved += ok
-        iters.append(info["iter"])
-    return {"value": solved / t_tot, "instances": n, "certified": solved, "iters": iters, "seconds": t_tot}
+        solved += abs(info["relgap"]) < 1e-5
+        iters.append(int(info["iter"]))
+    return {"value": solved / t_tot, "instances": pick, "certified": int(solved), "iters": iters,
+            "seconds": t_tot, "lams": [round(specs[b]["lam"], 3) for b in pick]}
 
 
-def cpu_baseline(bat, specs, budget_s):
-    """Oracle (float64 numpy/scipy, 1 thread) on instance 0: time per
+def cpu_baseline(bat, specs, budget_s, gpu_iters):
+    """Oracle (float64 numpy/scipy, one thread) on instance 0: seconds per
     iteration over a bounded sample, scaled by the iterations the GPU needed
     for that instance."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
@@ -122,12 +157,15 @@ def cpu_baseline(bat, specs, budget_s):
     opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
     dt = time.perf_counter() - t
     while dt < budget_s and n_it < 400:
-        n_it = max(n_it * 2, int(n_it * budget_s / max(dt, 1e-3)))
-        n_it = min(n_it, 400)
+        n_it = min(400, max(n_it * 2, int(n_it * budget_s / max(dt, 1e-3))))
         t = time.perf_counter()
         opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
         dt = time.perf_counter() - t
-    return dt / n_it, n_it, dt
+    per_it = dt / n_it
+    return {"value": 1.0 / (per_it * gpu_iters), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "oracle (float64 numpy/scipy, adaptive steps, 1 thread) on instance 0 (%dx%d): %d iterations "
+                      "in %.1fs, scaled to the %d iterations the GPU needed for that instance"
+                      % (H, W, n_it, dt, gpu_iters)}
 
 
 def main():
@@ -138,16 +176,19 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--max-iters", type=int, default=4000)
+    ap.add_argument("--max-iters", type=int, default=10000)
     ap.add_argument("--block-iters", type=int, default=50)
-    ap.add_argument("--tile", type=int, nargs=2, default=[64, 4])
-    ap.add_argument("--plain", type=int, default=2, help="instances for the plain single-instance baseline")
+    ap.add_argument("--tile", type=int, nargs=2, default=[128, 8])
+    ap.add_argument("--warm", action="store_true", help="start from the trained network's prediction")
+    ap.add_argument("--plain", type=int, default=6, help="instances for the plain single-instance baseline")
+    ap.add_argument("--reps", type=int, default=20, help="launches per pass in the roofline phase")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (instances would shard across "
-              "GPUs by bin-packing, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (batches would shard across GPUs by "
+              "instance, see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -156,11 +197,25 @@ def main():
     _hip.load()
     bat, specs = build_batch(args)
     npx = sum(h * w for h, w in bat.shapes)
+    lams = [s["lam"] for s in specs]
+
+    if args.roofline_only:
+        kt = roofline_phase(bat, args.reps)
+        log("roofline phase: dual %.4f ms, primal %.4f ms" % (kt["dual"], kt["primal"]))
+        return
+
+    warm = None
+    if args.warm:
+        from opymize_amd.warmstart.cnn import ConvNet, WarmStart
+        warm = WarmStart(ConvNet.load(os.path.join(ROOT, "opymize_amd", "warmstart", "weights_tv_l2.npz")))
 
     def one_step():
-        bat.set_start(None, None)
+        if warm is not None:
+            warm.predict_into(bat, lams)
+        else:
+            bat.set_start(None, None)
         return bat.solve(steps="adaptive", step_bound=0.124, max_iters=args.max_iters,
-                         block_iters=args.block_iters, tol=1e-5, certify=True)
+                         block_iters=args.block_iters, tol=1e-5, certify=True, fallback=warm is not None)
 
     for _ in range(args.warmup):
         one_step()
@@ -179,26 +234,20 @@ def main():
     log("batched: %d/%d certified in %.2fs, iterations min/median/max %d/%d/%d"
         % (certified, args.steps * args.batch, elapsed, iters.min(), np.median(iters), iters.max()))
 
-    kt = time_dominant_kernel(bat)
-    dual_bytes = 24.0 * npx  # reads x, x+, y (4+4+8 B), writes y+ (8 B) per pixel
-    achieved = dual_bytes / (kt["dual"] * 1e-3) / 1e9
-    log("dual pass %.3f ms (%.0f GB/s), primal pass %.3f ms (%.0f GB/s)"
-        % (kt["dual"], achieved, kt["primal"], 20.0 * npx / (kt["primal"] * 1e-3) / 1e9))
+    kt = roofline_phase(bat, args.reps)
+    achieved = DUAL_BYTES_PER_PX * npx / (kt["dual"] * 1e-3) / 1e9
+    log("roofline phase: dual %.3f ms (%.0f GB/s), primal %.3f ms (%.0f GB/s)"
+        % (kt["dual"], achieved, kt["primal"], PRIMAL_BYTES_PER_PX * npx / (kt["primal"] * 1e-3) / 1e9))
 
     plain = plain_single_instance(bat, specs, args.plain, args.max_iters) if args.plain > 0 else None
     if plain:
-        log("plain single-instance: %d/%d certified, %.3f solves/s, iterations %s"
-            % (plain["certified"], plain["instances"], plain["value"], plain["iters"]))
+        log("plain single-instance: %d/%d certified, %.3f solves/s, iterations %s, lambdas %s"
+            % (plain["certified"], len(plain["instances"]), plain["value"], plain["iters"], plain["lams"]))
 
     cpu = None
     if not args.no_cpu:
-        per_it, n_it, dt = cpu_baseline(bat, specs, args.cpu_budget)
-        it0 = int(iters[0]) if len(iters) else args.max_iters
-        cpu = {"value": 1.0 / (per_it * it0), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": "oracle (float64 numpy/scipy, adaptive steps) on instance 0 (1024x1024): %d iterations "
-                         "in %.1fs, scaled to the %d iterations the GPU needed for that instance"
-                         % (n_it, dt, it0)}
-        log("cpu oracle: %.3f s/iteration" % per_it)
+        cpu = cpu_baseline(bat, specs, args.cpu_budget, int(iters[0]))
+        log("cpu oracle: %s" % cpu["sample"])
 
     line = {
         "metric": "solves/sec to rel PD-gap 1e-5: batched+warm-start vs plain PDHG; %HBM peak",
@@ -208,14 +257,17 @@ def main():
         "config": {"workload": "TV-L2 denoising, %d instances of %dx%d per batch, rectangle generator, lambda "
                                "log-uniform [1,20], adaptive steps, rel gap 1e-5 FP64-certified, cap %d iterations"
                                % (args.batch, args.size, args.size, args.max_iters),
-                   "warm_start": "none", "tile": list(args.tile), "block_iters": args.block_iters,
+                   "warm_start": "cnn (6x3x3 conv, 32 ch, bf16 MFMA) with cold-start fallback" if warm else "none",
+                   "tile": list(args.tile), "block_iters": args.block_iters,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
-                   "iterations_median": float(np.median(iters)),
+                   "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max()),
                    "plain_single_instance_solves_per_sec": plain["value"] if plain else None,
+                   "plain_instances": len(plain["instances"]) if plain else 0,
                    "ratio_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "tv_dual_kernel<float,1,adaptive> (24 B/pixel algorithmic)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(npx, args.tile),
+                     "kernel": "tv_dual_kernel<float,1,8,adaptive>: %d B/pixel algorithmic x %d pixels per launch"
+                               % (DUAL_BYTES_PER_PX, npx),
                      "avg_ms": kt["dual"], "primal_avg_ms": kt["primal"]},
         "cpu_baseline": cpu,
     }
