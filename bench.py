@@ -11,9 +11,10 @@ Printed next to it:
   * the plain single-instance GPU PDHG (the drop-in generic kernel chain of
     PDHG.solve, cold start, same step rule and cap) on instances spread over
     the lambda range of the same batch;
-  * the roofline of the dominant kernel (the adaptive dual pass): algorithmic
-    bytes of a full-batch launch over the average duration of eager
-    full-batch launches, timed with HIP events on the launch stream;
+  * the roofline of the dominant kernel (the single-pass iteration kernel
+    with adaptive steps; the dual pass with --passes 2): algorithmic bytes of
+    a full-batch launch over the average duration of eager full-batch
+    launches, timed with HIP events on the launch stream;
   * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--warm]
@@ -31,8 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-DUAL_BYTES_PER_PX = 24.0    # adaptive dual pass, C=1 FP32: reads x_k, x_k+1 (4+4 B), y_k (8 B); writes y+ (8 B)
-PRIMAL_BYTES_PER_PX = 20.0  # primal pass: reads x_k, f (4+4 B), y_k (8 B); writes x_k+1 (4 B)
+ITER_BYTES_PER_PX = 28.0    # single pass, C=1 FP32: reads x_k, f (4+4 B), y_k (8 B); writes x_k+1 (4 B), y_k+1 (8 B)
+DUAL_BYTES_PER_PX = 24.0    # two passes: the dual pass reads x_k, x_k+1 (4+4 B), y_k (8 B); writes y_k+1 (8 B)
+PRIMAL_BYTES_PER_PX = 20.0  # two passes: the primal pass reads x_k, f (4+4 B), y_k (8 B); writes x_k+1 (4 B)
 
 
 def log(*a):
@@ -47,16 +49,18 @@ def build_batch(args):
     t = time.time()
     imgs = synth.images(specs)
     log("generated %d images of %dx%d in %.1fs" % (len(imgs), args.size, args.size, time.time() - t))
-    bat = TVBatch(imgs, [s["lam"] for s in specs], "l2", precision="float", tile=tuple(args.tile))
+    bat = TVBatch(imgs, [s["lam"] for s in specs], "l2", precision="float", tile=tuple(args.tile),
+                  passes=args.passes)
     del imgs
     torch.cuda.synchronize()
     return bat, specs
 
 
 def roofline_phase(bat, reps):
-    """Eager full-batch launches of the two passes (every instance running),
-    timed with HIP events on the stream they are launched on; one launch of
-    the library's copy kernel over a buffer of known size for counter scaling."""
+    """Eager full-batch launches of the iteration kernel(s) with adaptive
+    steps (every instance running), timed with HIP events on the stream they
+    are launched on; one launch of the library's copy kernel over a buffer of
+    known size for counter scaling."""
     import ctypes
     import torch
     from opymize_amd import _hip
@@ -68,7 +72,11 @@ def roofline_phase(bat, reps):
     s = _hip.stream()
 
     def launch(name):
-        if name == "dual":
+        if name == "iter":
+            bat._call("opy_tv_iter" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.f),
+                      P(bat.insts), P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact),
+                      ctypes.byref(cfg), 1, P(bat.part_res), bat.grid, s)
+        elif name == "dual":
             bat._call("opy_tv_dual" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.insts),
                       P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact), ctypes.byref(cfg), 1,
                       P(bat.part_res), bat.grid, s)
@@ -77,7 +85,7 @@ def roofline_phase(bat, reps):
                       P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact), ctypes.byref(cfg), bat.grid, s)
 
     out = {}
-    for name in ("dual", "primal"):
+    for name in (("iter",) if bat.passes == 1 else ("dual", "primal")):
         launch(name)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -96,11 +104,13 @@ def roofline_phase(bat, reps):
     return out
 
 
-def measured_traffic(npx, tile):
-    """HBM bytes per full-batch dual launch from the committed counter runs
-    (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of
-    known size), when they were taken on this workload; else None."""
-    path = os.path.join(ROOT, "profiles", "tv_dual_kernel_tvl2_256x1024_counters.json")
+def measured_traffic(npx, tile, passes):
+    """HBM bytes per full-batch launch of the dominant kernel from the
+    committed counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it
+    reads for a copy of known size), when they were taken on this workload;
+    else None."""
+    name = "tv_iter_kernel" if passes == 1 else "tv_dual_kernel"
+    path = os.path.join(ROOT, "profiles", "%s_tvl2_256x1024_counters.json" % name)
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -178,7 +188,8 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--max-iters", type=int, default=10000)
     ap.add_argument("--block-iters", type=int, default=50)
-    ap.add_argument("--tile", type=int, nargs=2, default=[128, 8])
+    ap.add_argument("--tile", type=int, nargs=2, default=[64, 8])
+    ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
     ap.add_argument("--warm", action="store_true", help="start from the trained network's prediction")
     ap.add_argument("--plain", type=int, default=6, help="instances for the plain single-instance baseline")
     ap.add_argument("--reps", type=int, default=20, help="launches per pass in the roofline phase")
@@ -201,7 +212,7 @@ def main():
 
     if args.roofline_only:
         kt = roofline_phase(bat, args.reps)
-        log("roofline phase: dual %.4f ms, primal %.4f ms" % (kt["dual"], kt["primal"]))
+        log("roofline phase: %s" % ", ".join("%s %.4f ms" % kv for kv in kt.items()))
         return
 
     warm = None
@@ -235,9 +246,15 @@ def main():
         % (certified, args.steps * args.batch, elapsed, iters.min(), np.median(iters), iters.max()))
 
     kt = roofline_phase(bat, args.reps)
-    achieved = DUAL_BYTES_PER_PX * npx / (kt["dual"] * 1e-3) / 1e9
-    log("roofline phase: dual %.3f ms (%.0f GB/s), primal %.3f ms (%.0f GB/s)"
-        % (kt["dual"], achieved, kt["primal"], PRIMAL_BYTES_PER_PX * npx / (kt["primal"] * 1e-3) / 1e9))
+    if bat.passes == 1:
+        kname, kms, kbytes = "iter", kt["iter"], ITER_BYTES_PER_PX
+        kdesc = "tv_iter_kernel<float,1,%d,adaptive> (one pass per iteration)" % args.tile[1]
+    else:
+        kname, kms, kbytes = "dual", kt["dual"], DUAL_BYTES_PER_PX
+        kdesc = "tv_dual_kernel<float,1,%d,adaptive> (second of two passes)" % args.tile[1]
+    achieved = kbytes * npx / (kms * 1e-3) / 1e9
+    log("roofline phase: %s -- %s %.0f GB/s algorithmic" % (
+        ", ".join("%s %.3f ms" % kv for kv in kt.items()), kname, achieved))
 
     plain = plain_single_instance(bat, specs, args.plain, args.max_iters) if args.plain > 0 else None
     if plain:
@@ -265,10 +282,9 @@ def main():
                    "plain_instances": len(plain["instances"]) if plain else 0,
                    "ratio_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(npx, args.tile),
-                     "kernel": "tv_dual_kernel<float,1,8,adaptive>: %d B/pixel algorithmic x %d pixels per launch"
-                               % (DUAL_BYTES_PER_PX, npx),
-                     "avg_ms": kt["dual"], "primal_avg_ms": kt["primal"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(npx, args.tile, args.passes),
+                     "kernel": "%s: %d B/pixel algorithmic x %d pixels per launch" % (kdesc, kbytes, npx),
+                     "avg_ms": kms, "launch_ms": kt},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OPY_ABI_VERSION 2
+#define OPY_ABI_VERSION 3
 #define OPY_MAX_DIMS 6
 #define OPY_RED_BLOCKS 1024   /* reductions need work buffers of 3*OPY_RED_BLOCKS doubles */
 #define OPY_QUADEPI_MAXM 16
@@ -272,7 +272,7 @@ typedef struct {
 
 typedef struct {
   int32_t C;      /* channels, 1..4 */
-  int32_t tw;     /* tile width; divides 256 */
+  int32_t tw;     /* threads per tile row; divides 256 */
   int32_t rpt;    /* rows per thread; tile height th = 256/tw*rpt */
   int32_t steps;  /* OPY_TV_STEPS_* */
   double h0, h1;  /* grid spacing */
@@ -302,6 +302,18 @@ int opy_tv_dual_f32(const float* x_old, const float* x_new, const float* y_in, f
                     const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
                     double* partials, int grid, void* stream);
 int opy_tv_dual_f64(const double* x_old, const double* x_new, const double* y_in, double* y_out,
+                    const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                    const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
+                    double* partials, int grid, void* stream);
+/* one whole iteration in one pass (cfg->tw = 64, cfg->rpt <= 16):
+ * x_out = prox_{tau G}(x_in - tau A^T y_in), y_out = proj(y_in + sigma A(2 x_out - x_in));
+ * reads x, f, y and writes x+, y+ (PDHG.iteration_step, pdhg.py:124-153, for TV models);
+ * adapt=1 also writes the 4 residual partials per tile (pdhg.py:155-164) */
+int opy_tv_iter_f32(const float* x_in, float* x_out, const float* y_in, float* y_out, const float* f,
+                    const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                    const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
+                    double* partials, int grid, void* stream);
+int opy_tv_iter_f64(const double* x_in, double* x_out, const double* y_in, double* y_out, const double* f,
                     const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
                     const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
                     double* partials, int grid, void* stream);

@@ -2,9 +2,10 @@
 
 One engine serves both the drop-in single-instance path (PDHG.solve on a
 recognised TV-L2/TV-L1 model) and the batched solver over many instances of
-uneven size.  Per iteration it launches two fused passes (and, with adaptive
-steps, one small per-instance rule kernel); every `block_iters` iterations it
-evaluates the FP64 primal-dual gap, marks converged instances and compacts
+uneven size.  Per iteration it launches one fused pass (two with passes=2)
+and, with adaptive steps, one small per-instance rule kernel; every
+`block_iters` iterations it evaluates the FP64 primal-dual gap, marks
+converged instances and compacts
 the list of running instances -- all on the device, so a whole block is one
 hipGraph replay and the host only reads the number of running instances.
 
@@ -48,6 +49,7 @@ assert ctypes.sizeof(TVInst) == 48 and ctypes.sizeof(TVCfg) == 64 and ctypes.siz
 _SIGS = {
     "opy_tv_primal": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_dual": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_iter": [ctypes.c_void_p] * 11 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_gap": [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_reset": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4,
 }
@@ -91,10 +93,13 @@ class TVBatch(object):
     lam:    data weight per instance (scalar or list)
     data:   "l2" / "l1" per instance (str or list)
     tvw:    TV weight per instance (scalar or list)
+    passes: 1 = one kernel per iteration (reads x, f, y; writes x+, y+),
+            2 = a primal and a dual pass (reads x, f, y, then x, x+, y)
+    tile:   (tile width, rows per thread); with passes=1 the width is 64
     """
 
     def __init__(self, images, lam, data="l2", tvw=1.0, precision="float", tile=(64, 4), h=(1.0, 1.0),
-                 weights=None, grid=None):
+                 weights=None, grid=None, passes=1):
         _hip.require_device()
         _bind()
         self.B = len(images)
@@ -130,11 +135,16 @@ class TVBatch(object):
         self.x = [torch.zeros(o, dtype=self.dtype, device="cuda") for _ in range(2)]
         self.y = [torch.zeros(2 * o, dtype=self.dtype, device="cuda") for _ in range(2)]
         self._grid_override = grid
+        if passes not in (1, 2):
+            raise ValueError("passes must be 1 or 2")
+        self.passes = passes
         self.set_tile(*tile)
         self.graph = None
 
     # -- layout --------------------------------------------------------------
     def set_tile(self, tw, rpt):
+        if self.passes == 1:
+            tw = 64
         th = tile_height(tw, rpt)
         self.tw, self.rpt = int(tw), int(rpt)
         insts = (TVInst * self.B)()
@@ -234,11 +244,16 @@ class TVBatch(object):
         s = _hip.stream()
         xo, xn = self.x[p], self.x[1 - p]
         yo, yn = self.y[p], self.y[1 - p]
-        self._call("opy_tv_primal" + self._sfx(), P(xo), P(xn), P(yo), P(self.f), P(self.insts), P(self.state),
-                   P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), self.grid, s)
-        self._call("opy_tv_dual" + self._sfx(), P(xo), P(xn), P(yo), P(yn), P(self.insts), P(self.state),
-                   P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), int(adapt),
-                   P(self.part_res), self.grid, s)
+        if self.passes == 1:
+            self._call("opy_tv_iter" + self._sfx(), P(xo), P(xn), P(yo), P(yn), P(self.f), P(self.insts),
+                       P(self.state), P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg),
+                       int(adapt), P(self.part_res), self.grid, s)
+        else:
+            self._call("opy_tv_primal" + self._sfx(), P(xo), P(xn), P(yo), P(self.f), P(self.insts), P(self.state),
+                       P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), self.grid, s)
+            self._call("opy_tv_dual" + self._sfx(), P(xo), P(xn), P(yo), P(yn), P(self.insts), P(self.state),
+                       P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), int(adapt),
+                       P(self.part_res), self.grid, s)
         if adapt:
             self._call("opy_tv_adapt", P(self.insts), P(self.state), P(self.checks), P(self.part_res), self.B,
                        self.C, ctypes.byref(params), s)

@@ -21,19 +21,20 @@ def rects(H, W, seed, nrect=6):
 SHAPES = [(17, 30), (40, 23), (5, 64), (64, 5), (1, 33), (70, 70)]
 
 
-def make_batch(precision, tile=(64, 2)):
+def make_batch(precision, tile=(64, 2), passes=1):
     from opymize_amd.solvers.tv import TVBatch
     imgs = [rects(H, W, i) for i, (H, W) in enumerate(SHAPES)]
     lams = [2.0, 7.5, 1.0, 15.0, 4.0, 3.3]
     data = ["l2", "l1", "l2", "l1", "l2", "l2"]
-    return TVBatch(imgs, lams, data, precision=precision, tile=tile), imgs, lams, data
+    return TVBatch(imgs, lams, data, precision=precision, tile=tile, passes=passes), imgs, lams, data
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("steps", ["adaptive", "const", "precond"])
-@pytest.mark.parametrize("tile", [(64, 2), (32, 1), (16, 4)])
-def test_batch_iterates_match_oracle(gpu, precision, steps, tile):
-    bat, imgs, lams, data = make_batch(precision, tile)
+@pytest.mark.parametrize("passes,tile", [(2, (64, 2)), (2, (32, 1)), (2, (16, 4)), (1, (64, 1)), (1, (64, 4)),
+                                          (1, (64, 8))])
+def test_batch_iterates_match_oracle(gpu, precision, steps, passes, tile):
+    bat, imgs, lams, data = make_batch(precision, tile, passes)
     k = 40
     bat.solve(steps=steps, step_bound=0.124, max_iters=k, block_iters=20, tol=1e-30)
     xs, ys = bat.solution()
@@ -103,9 +104,9 @@ def test_batch_warm_start_from_solution(gpu):
 
 @pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("steps", ["adaptive", "precond"])
-@pytest.mark.parametrize("tile", [(128, 8), (256, 16), (64, 4)])
+@pytest.mark.parametrize("passes,tile", [(2, (128, 8)), (2, (256, 16)), (2, (64, 4)), (1, (64, 2)), (1, (64, 8))])
 @pytest.mark.parametrize("C", [1, 2, 3, 4])
-def test_batch_channels_and_wide_tiles(gpu, precision, steps, tile, C):
+def test_batch_channels_and_wide_tiles(gpu, precision, steps, passes, tile, C):
     """Multi-channel batches (per-pixel vectors of C and 2C values) and tiles
     whose rows span several wavefronts, against the oracle after 30 iterations."""
     from opymize_amd.solvers.tv import TVBatch
@@ -116,7 +117,7 @@ def test_batch_channels_and_wide_tiles(gpu, precision, steps, tile, C):
     imgs = [np.stack([rects(H, W, 10 * C + i + k) for k in range(C)], axis=2) for i, (H, W) in enumerate(shapes)]
     lams = [3.0, 9.0, 1.5]
     data = ["l2", "l1", "l2"]
-    bat = TVBatch(imgs, lams, data, precision=precision, tile=tile)
+    bat = TVBatch(imgs, lams, data, precision=precision, tile=tile, passes=passes)
     k = 30
     bat.solve(steps=steps, step_bound=0.124, max_iters=k, block_iters=10, tol=1e-30)
     xs, ys = bat.solution()
