@@ -109,3 +109,48 @@ def test_training_reduces_loss(gpu):
     xd, td = torch.as_tensor(x).cuda(), torch.as_tensor(tgt).cuda()
     losses = [net.train_step(xd, td, lr=3e-3).sum().item() for _ in range(300)]
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+
+
+def test_trained_network_gain_over_cold_start(gpu):
+    """What the committed network (trained on solver outputs of the rectangle
+    generator, seed 11) gains on held-out instances (seed 12): a start point
+    with a finite, small relative gap (cold: y = 0 gives a dual value near 0),
+    and fewer iterations to a relative gap of 1e-2.  Both runs then reach the
+    same solution: ||x_warm - x_cold|| <= sqrt(2 g_w / lam) + sqrt(2 g_c / lam)
+    (strong convexity of the SSD term)."""
+    import os
+    from opymize_amd.solvers.tv import TVBatch
+    from opymize_amd.tools import synth
+    from opymize_amd.warmstart.cnn import ConvNet, WarmStart
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "opymize_amd", "warmstart",
+                        "weights_tv_l2.npz")
+    specs = synth.batch_square(8, side=256, seed=12)
+    imgs = synth.images(specs)
+    lams = [s["lam"] for s in specs]
+    ws = WarmStart(ConvNet.load(path))
+    out = {}
+    for mode in ("cold", "warm"):
+        bat = TVBatch(imgs, lams, "l2", precision="float", tile=(64, 8))
+        if mode == "warm":
+            ws.predict_into(bat, lams)
+        r0 = bat.solve(steps="adaptive", step_bound=0.124, tol=1e-30, max_iters=0, check_start=True)
+        first = np.full(len(imgs), np.inf)
+
+        def cb(b, it):
+            rg = np.array([abs(q["relgap"]) for q in b.results()])
+            first[(rg < 1e-2) & ~np.isfinite(first)] = it
+            return bool(np.all(np.isfinite(first)) and it >= 2000)
+
+        bat.solve(steps="adaptive", step_bound=0.124, tol=1e-30, max_iters=2000, block_iters=50, callback=cb)
+        res = bat.results()
+        out[mode] = (np.array([abs(q["relgap"]) for q in r0]), first, bat.solution()[0],
+                     np.array([q["gap"] for q in res]))
+    start_c, first_c, x_c, gap_c = out["cold"]
+    start_w, first_w, x_w, gap_w = out["warm"]
+    assert np.all(start_w < 10.0) and np.all(start_c > 1e6)
+    assert np.all(np.isfinite(first_w)) and np.all(np.isfinite(first_c))
+    assert np.median(first_w) < np.median(first_c)
+    assert first_w.sum() < first_c.sum()
+    for b, lam in enumerate(lams):
+        bound = np.sqrt(2 * max(gap_w[b], 0) / lam) + np.sqrt(2 * max(gap_c[b], 0) / lam)
+        assert np.linalg.norm(x_w[b] - x_c[b]) <= bound * 1.001 + 1e-3
