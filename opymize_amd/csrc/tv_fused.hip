@@ -644,19 +644,23 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         load_c<T, C>(fb + ((int64_t)rx * W + cl) * C, flh);
       }
     }
-    // ---- x+ (and A^T y_k, reused by the residual) on the segment rows
-    T xp[NX][C], aty[NX][C];
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      const int r = rs - OX + j;
+    // ---- x+ on the segment rows; A^T y_k at row j (recomputed by the residual)
+    auto aty_row = [&](int j, T* out) {
       T yl1[C];
 #pragma unroll
       for (int k = 0; k < C; ++k) yl1[k] = dpp_shr1(readlane(ylh[C + k], j + 1), yv[j + 1][C + k]);
-      adjoint_px<T, C>(yv[j + 1], yv[j + 1] + C, yv[j], yl1, r, c, H, W, cfg, aty[j]);
+      adjoint_px<T, C>(yv[j + 1], yv[j + 1] + C, yv[j], yl1, rs - OX + j, c, H, W, cfg, out);
+    };
+    T xp[NX][C];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      const int r = rs - OX + j;
+      T aty[C];
+      aty_row(j, aty);
 #pragma unroll
       for (int k = 0; k < C; ++k) {
         const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(r, c, H, W, k, cfg) : (T)tau_s;
-        xp[j][k] = prox_data<T>(xv[j][k] - tau * aty[j][k], fv[j][k], (double)tau * I.lam, I.data);
+        xp[j][k] = prox_data<T>(xv[j][k] - tau * aty[k], fv[j][k], (double)tau * I.lam, I.data);
       }
     }
     // ---- right halo, transposed: lane k forms x+ at (x row k, cr)
@@ -684,19 +688,9 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         xdh[k] = xrh[k] - p;
       }
     }
-    // ---- xbar = 2x+ - x, xd = x - x+ and their right neighbours
-    T xb_[NX][C], xbr[NX][C], xd[NX][C], xdr[NX][C];
-#pragma unroll
-    for (int j = 0; j < NX; ++j)
-#pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xb_[j][k] = T(2) * xp[j][k] - xv[j][k];
-        xbr[j][k] = dpp_shl1(readlane(xbh[k], j), xb_[j][k]);
-        if (kAdapt) {
-          xd[j][k] = xv[j][k] - xp[j][k];
-          xdr[j][k] = dpp_shl1(readlane(xdh[k], j), xd[j][k]);
-        }
-      }
+    // xbar = 2x+ - x and xd = x - x+ (and their right neighbours) are formed
+    // where they are used, which keeps fewer values live
+    auto xbar = [&](int j, int k) { return T(2) * xp[j][k] - xv[j][k]; };
     // ---- left halo (adaptive), transposed: lane k forms y+ at (y+ row k, cl), t=1 part
     T yplh[C];
 #pragma unroll
@@ -716,10 +710,10 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
 #pragma unroll
       for (int k = 0; k < C; ++k) {
         xbld[k] = dpp_shl1(T(0), xbl[k]);  // x row k+1
-        T sel = T(0);                      // xbar(x row k, c0), held by lane 0 as xb_[k]
+        T sel = T(0);                      // xbar(x row k, c0), formed by lane 0
 #pragma unroll
         for (int j = 0; j < NX; ++j) {
-          const T v = readlane(xb_[j][k], 0);
+          const T v = readlane(xbar(j, k), 0);
           sel = lane == j ? v : sel;
         }
         xbc[k] = sel;
@@ -739,8 +733,14 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
       const int r = rs - OX + i;
       const bool mine = i >= OX;
       const bool ok = col && r >= 0 && r < H;
-      T yp[2 * C], gb[2 * C];
-      yplus_px<T, C>(xb_[i], xb_[i + 1], xbr[i], yv[i + 1], r, c, H, W, sigma, tvw, cfg, yp, gb);
+      T yp[2 * C], gb[2 * C], xbi[C], xbd[C], xbr[C];
+#pragma unroll
+      for (int k = 0; k < C; ++k) {
+        xbi[k] = xbar(i, k);
+        xbd[k] = xbar(i + 1, k);
+        xbr[k] = dpp_shl1(readlane(xbh[k], i), xbi[k]);
+      }
+      yplus_px<T, C>(xbi, xbd, xbr, yv[i + 1], r, c, H, W, sigma, tvw, cfg, yp, gb);
       if (mine && ok) {
         const int64_t p = (int64_t)r * W + c;
         store_c<T, C>(xob + p * C, xp[i]);
@@ -753,12 +753,21 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         if (mine && ok) {
           // primal residual (x - x+)/tau - A^T(y - y+) with its reference term,
           // dual residual (y - y+)/sigma - A(x - x+) and (y - y+)/sigma + A xbar
-          T atn[C], gxd[2 * C];
-          adjoint_px<T, C>(yp, yp + C, ypu, ypl1, r, c, H, W, cfg, atn);
-          grad_px<T, C>(xd[i], xd[i + 1], xdr[i], r, c, H, W, cfg, gxd);
+          T atn[C], gxd[2 * C], xdi[C], xdd[C], xdr[C];
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            const T q = xd[i][k] * itau - aty[i][k];
+            xdi[k] = xv[i][k] - xp[i][k];
+            xdd[k] = xv[i + 1][k] - xp[i + 1][k];
+          }
+          T aty[C];
+          aty_row(i, aty);
+          adjoint_px<T, C>(yp, yp + C, ypu, ypl1, r, c, H, W, cfg, atn);
+#pragma unroll
+          for (int k = 0; k < C; ++k) xdr[k] = dpp_shl1(readlane(xdh[k], i), xdi[k]);
+          grad_px<T, C>(xdi, xdd, xdr, r, c, H, W, cfg, gxd);
+#pragma unroll
+          for (int k = 0; k < C; ++k) {
+            const T q = xdi[k] * itau - aty[k];
             const T rp = q + atn[k];
             s[0] += (double)rp * (double)rp;
             s[1] += (double)q * (double)q;
@@ -1073,36 +1082,30 @@ int check_cfg(const opy_tv_cfg_t* c) {
   if (!c) return fail(kInvalid, "tv: null config");
   if (c->C < 1 || c->C > 4) return fail(kInvalid, "tv: C=%d outside [1,4]", c->C);
   if (c->tw < 1 || c->tw > kTB || kTB % c->tw) return fail(kInvalid, "tv: tile width %d must divide %d", c->tw, kTB);
-  if (c->rpt != 1 && c->rpt != 2 && c->rpt != 4 && c->rpt != 8 && c->rpt != 16)
-    return fail(kInvalid, "tv: rows per thread %d not in {1,2,4,8,16}", c->rpt);
+  if (c->rpt != 1 && c->rpt != 2 && c->rpt != 4 && c->rpt != 6 && c->rpt != 8 && c->rpt != 12 && c->rpt != 16)
+    return fail(kInvalid, "tv: rows per thread %d not in {1,2,4,6,8,12,16}", c->rpt);
   if (c->steps != OPY_TV_STEPS_SCALAR && c->steps != OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv: bad step mode");
   if (!(c->h0 > 0) || !(c->h1 > 0)) return fail(kInvalid, "tv: grid spacing must be positive");
   return kOk;
 }
 
-// dispatch a kernel template over C (1..4) and RPT (1,2,4,8,16)
-#define OPY_TV_DISPATCH(KERNEL_EXPR_C_R)                                    \
-  switch (cfg->C * 100 + cfg->rpt) {                                        \
-    case 101: KERNEL_EXPR_C_R(1, 1); break;                                 \
-    case 102: KERNEL_EXPR_C_R(1, 2); break;                                 \
-    case 104: KERNEL_EXPR_C_R(1, 4); break;                                 \
-    case 108: KERNEL_EXPR_C_R(1, 8); break;                                 \
-    case 116: KERNEL_EXPR_C_R(1, 16); break;                                \
-    case 201: KERNEL_EXPR_C_R(2, 1); break;                                 \
-    case 202: KERNEL_EXPR_C_R(2, 2); break;                                 \
-    case 204: KERNEL_EXPR_C_R(2, 4); break;                                 \
-    case 208: KERNEL_EXPR_C_R(2, 8); break;                                 \
-    case 216: KERNEL_EXPR_C_R(2, 16); break;                                \
-    case 301: KERNEL_EXPR_C_R(3, 1); break;                                 \
-    case 302: KERNEL_EXPR_C_R(3, 2); break;                                 \
-    case 304: KERNEL_EXPR_C_R(3, 4); break;                                 \
-    case 308: KERNEL_EXPR_C_R(3, 8); break;                                 \
-    case 316: KERNEL_EXPR_C_R(3, 16); break;                                \
-    case 401: KERNEL_EXPR_C_R(4, 1); break;                                 \
-    case 402: KERNEL_EXPR_C_R(4, 2); break;                                 \
-    case 404: KERNEL_EXPR_C_R(4, 4); break;                                 \
-    case 408: KERNEL_EXPR_C_R(4, 8); break;                                 \
-    default: KERNEL_EXPR_C_R(4, 16); break;                                 \
+// dispatch a kernel template over C (1..4) and RPT (1,2,4,6,8,12,16)
+#define OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, CC)       \
+  switch (cfg->rpt) {                                \
+    case 1: KERNEL_EXPR_C_R(CC, 1); break;           \
+    case 2: KERNEL_EXPR_C_R(CC, 2); break;           \
+    case 4: KERNEL_EXPR_C_R(CC, 4); break;           \
+    case 6: KERNEL_EXPR_C_R(CC, 6); break;           \
+    case 8: KERNEL_EXPR_C_R(CC, 8); break;           \
+    case 12: KERNEL_EXPR_C_R(CC, 12); break;         \
+    default: KERNEL_EXPR_C_R(CC, 16); break;         \
+  }
+#define OPY_TV_DISPATCH(KERNEL_EXPR_C_R)                     \
+  switch (cfg->C) {                                          \
+    case 1: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 1); break;    \
+    case 2: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 2); break;    \
+    case 3: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 3); break;    \
+    default: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 4); break;   \
   }
 
 template <typename T>
