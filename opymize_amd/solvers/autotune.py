@@ -12,15 +12,20 @@ import time
 import numpy as np
 import torch
 
-TILE_WIDTHS = (32, 64, 128, 256)
+TILE_WIDTHS = (32, 64, 128, 256)          # two-pass kernels
 ROWS_PER_THREAD = (1, 2, 4, 8)
+ITER_ROWS_PER_THREAD = (4, 6, 8, 12, 16)  # single-pass kernel (tile width 64)
 BLOCK_ITERS = (10, 20, 50, 100)
 
 _cache = {}
 
 
-def default_tile(H, W, C):
-    """Full wavefront rows for wide images, narrower tiles for thin ones."""
+def default_tile(H, W, C, passes=1):
+    """Single pass: 64 columns, 8 rows per thread for C=1 and 4 beyond (register
+    budget).  Two passes: full wavefront rows for wide images, narrower tiles
+    for thin ones."""
+    if passes == 1:
+        return 64, (8 if C == 1 else 4)
     tw = 64
     while tw > 8 and tw // 2 >= W:
         tw //= 2
@@ -30,13 +35,17 @@ def default_tile(H, W, C):
 
 def _key(batch):
     px = sum(h * w for h, w in batch.shapes)
-    return (batch.C, str(batch.dtype), int(np.log2(max(px, 1))), batch.B > 1)
+    return (batch.C, str(batch.dtype), int(np.log2(max(px, 1))), batch.B > 1, batch.passes)
 
 
-def tune(batch, steps="adaptive", step_bound=0.124, iters=20, widths=TILE_WIDTHS, rpts=ROWS_PER_THREAD,
+def tune(batch, steps="adaptive", step_bound=0.124, iters=20, widths=None, rpts=None,
          blocks=BLOCK_ITERS, path=None, verbose=False):
     """Pick the fastest (tw, rpt) for `batch` and the graph block length;
     leaves the batch set to the winner and returns {"tile": ..., "block_iters": ...}."""
+    if widths is None:
+        widths = (64,) if batch.passes == 1 else TILE_WIDTHS
+    if rpts is None:
+        rpts = ITER_ROWS_PER_THREAD if batch.passes == 1 else ROWS_PER_THREAD
     key = _key(batch)
     if key in _cache:
         best = _cache[key]

@@ -129,3 +129,26 @@ def test_batch_channels_and_wide_tiles(gpu, precision, steps, passes, tile, C):
                                   term_pd_gap=1e-30)
         assert np.abs(xs[b] - xo).max() < tol, b
         assert np.abs(ys[b] - yo).max() < tol, b
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+def test_autotune_picks_a_candidate_and_keeps_results(gpu, passes):
+    """The tuner times the candidate tiles on the batch itself, restores the
+    start point and leaves the winner set; iterates afterwards still match
+    the oracle."""
+    from opymize_amd.solvers import autotune
+    bat, imgs, lams, data = make_batch("double", (64, 2), passes)
+    autotune._cache.clear()
+    best = autotune.tune(bat, iters=4, blocks=(10, 20))
+    cands = autotune.ITER_ROWS_PER_THREAD if passes == 1 else autotune.ROWS_PER_THREAD
+    assert best["tile"][1] in cands and (bat.tw, bat.rpt) == tuple(best["tile"])
+    assert best["block_iters"] in (10, 20)
+    k = 20
+    bat.solve(steps="adaptive", step_bound=0.124, max_iters=k, block_iters=10, tol=1e-30)
+    xs, _ = bat.solution()
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, _, _, _ = opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=k, granularity=k,
+                                term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < 1e-9, b
