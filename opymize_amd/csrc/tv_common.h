@@ -1,0 +1,342 @@
+// Shared helpers of the fused TV kernels (tv_passes.hip, tv_iter.hip,
+// tv_control.hip): configuration, tile map, vector loads and streaming
+// stores, per-pixel stencils, cross-lane moves and the host-side launch
+// helpers.  Everything sits in an unnamed namespace: each translation
+// unit gets its own copy.
+//
+// Fused PDHG iterations for TV-regularised 2-D denoising, batched over many
+// instances of uneven size.
+//
+// Model per instance b (image H x W with C channels, forward differences):
+//   min_x  G_b(x) + tvw_b * sum_p |grad x (p)|_F
+//   G_b = 0.5*lam_b*|x - f|^2 (SSD, reference functionals/quad.py) or
+//         lam_b*|x - f|_1     (TV-L1 data term)
+// i.e. PDHG(G, L1Norms(N, (C, 2), tvw), GradientOp(scheme="forward")) of the
+// reference (pdhg.py:124-187, diff.cu:64-71, :114-123, proj.cu:34-44).
+//
+// One iteration is two passes over HBM:
+//   pass 1 (primal): x+ = prox_{tau G}(x - tau * A^T y)           reads x, y, f; writes x+
+//   pass 2 (dual):   y+ = proj_{|.|<=tvw}(y + sigma * A(2x+ - x))  reads x, x+, y; writes y+
+// The over-relaxed A x_bar = 2 A x+ - A x of the reference is applied as
+// A(2x+ - x) (equal by linearity).  With adaptive steps pass 2 also forms the
+// four residual norms of pdhg.py:155-164.
+//
+// Thread layout: a tile is tw columns x (256/tw)*RPT rows; a thread owns one
+// column segment of RPT consecutive rows and marches down it, carrying the
+// row above / below in registers.  Horizontal neighbours come from the
+// neighbouring lane by cross-lane shuffles; only the first/last lane of a row
+// segment of a wavefront loads (or, for y+, recomputes) its neighbour.
+//
+// A persistent grid walks the tiles of the still-running instances:
+// active_inst / act_off hold the running instances and the exclusive prefix
+// of their tile counts; block k takes tiles k, k + grid, k + 2 grid, ... so
+// that the blocks running at one time work on neighbouring tiles (DRAM page
+// locality, shared halos; contiguous chunks per block measured slower).
+// Blocks are renumbered so that consecutive tiles run on the same XCD (the
+// dispatcher deals blocks round-robin to the 8 XCDs, each with its own L2),
+// which keeps the halo rows between tiles in one L2.
+// The single-pass kernel (tv_iter_kernel) forms its halos "transposed", see
+// its comment.  All
+// per-instance reductions (residuals, gap) go through per-tile FP64 partials
+// that a per-instance kernel folds in a fixed order (reproducible results).
+#pragma once
+
+#include "opy_common.h"
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <type_traits>
+#include <utility>
+
+namespace opy {
+namespace {
+
+constexpr int kTB = 256;
+constexpr double kEps = 2.220446049250313e-16;
+
+struct Cfg {
+  int C, tw, rpt, steps;
+  double ih0, ih1;
+  double w[4];
+};
+
+inline Cfg make_cfg(const opy_tv_cfg_t* c) {
+  Cfg r;
+  r.C = c->C;
+  r.tw = c->tw;
+  r.rpt = c->rpt;
+  r.steps = c->steps;
+  r.ih0 = 1.0 / c->h0;
+  r.ih1 = 1.0 / c->h1;
+  for (int k = 0; k < 4; ++k) r.w[k] = c->w[k];
+  return r;
+}
+
+struct TileRef {
+  int b;  // instance
+  int j;  // tile index within the instance
+  int r0, c0;
+};
+
+__device__ inline int xcd_block() {
+  // blocks are dealt round-robin to the 8 XCDs: give each XCD a contiguous range
+  const int g = gridDim.x;
+  return (g % 8 == 0) ? (blockIdx.x % 8) * (g / 8) + blockIdx.x / 8 : blockIdx.x;
+}
+
+// tile g of the running instances -> (instance, tile index, first row, first
+// column): binary search of g in act_off; tiles are tw image columns by th
+// rows, numbered row by row within an instance
+__device__ inline TileRef locate(int g, const int32_t* __restrict__ active_inst, const int32_t* __restrict__ act_off,
+                                 int nact, const opy_tv_inst_t* __restrict__ insts, int tw, int th) {
+  int lo = 0, hi = nact - 1;  // largest a with act_off[a] <= g
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (act_off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  TileRef t;
+  t.b = active_inst[lo];
+  t.j = g - act_off[lo];
+  const int W = insts[t.b].W;
+  const int ntx = (W + tw - 1) / tw;
+  const int ty = t.j / ntx;
+  t.r0 = ty * th;
+  t.c0 = (t.j - ty * ntx) * tw;
+  return t;
+}
+
+template <typename T>
+__device__ inline T shfl_up1(T v) { return __shfl_up(v, 1, 64); }
+template <typename T>
+__device__ inline T shfl_down1(T v) { return __shfl_down(v, 1, 64); }
+
+template <typename T>
+__device__ inline T precond_tau(int r, int c, int H, int W, int k, const Cfg& cfg) {
+  // 1 / column sum of |A| (reference pdhg.py:197-200 via rowwise_lp of the divergence)
+  const double s = cfg.w[k] * (((r < H - 1) + (r > 0)) * cfg.ih0 + ((c < W - 1) + (c > 0)) * cfg.ih1);
+  return (T)(s > kEps ? 1.0 / s : s);
+}
+
+template <typename T>
+__device__ inline T precond_sigma(bool inner, int k, double ih, const Cfg& cfg) {
+  const double s = inner ? 2.0 * cfg.w[k] * ih : 0.0;
+  return (T)(s > kEps ? 1.0 / s : s);
+}
+
+template <typename T>
+__device__ inline T prox_data(T v, T f, double tl, int data) {
+  if (data == OPY_TV_DATA_L1) {
+    const T thr = (T)tl;
+    const T d = v - f;
+    return f + (d > thr ? d - thr : (d < -thr ? d + thr : T(0)));
+  }
+  const T a = (T)(1.0 / (1.0 + tl));
+  const T b = (T)tl;
+  return a * (v + b * f);
+}
+
+// N consecutive values from an address aligned to min(N, 4) * sizeof(T) for
+// even N (per-pixel vectors inside instance blocks whose offsets are
+// multiples of the vector length): one vector load instead of N scalar ones.
+template <typename T, int N>
+__device__ inline void load_c(const T* __restrict__ p, T* v) {
+  if constexpr (sizeof(T) == 4 && N % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k += 4) {
+      const float4 q = *reinterpret_cast<const float4*>(p + k);
+      v[k] = q.x;
+      v[k + 1] = q.y;
+      v[k + 2] = q.z;
+      v[k + 3] = q.w;
+    }
+  } else if constexpr (N % 2 == 0) {
+    using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const V2 q = *reinterpret_cast<const V2*>(p + k);
+      v[k] = q.x;
+      v[k + 1] = q.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = p[k];
+  }
+}
+
+// Streaming stores of the new iterates (nontemporal: at batch sizes beyond
+// the last-level cache they are not re-read from it), vectorised like load_c.
+template <typename T, int N>
+__device__ inline void store_c(T* __restrict__ p, const T* v) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  if constexpr (sizeof(T) == 4 && N % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k += 4) {
+      const v4 q = {v[k], v[k + 1], v[k + 2], v[k + 3]};
+      __builtin_nontemporal_store(q, reinterpret_cast<v4*>(p + k));
+    }
+  } else if constexpr (N % 2 == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+      const v2 q = {v[k], v[k + 1]};
+      __builtin_nontemporal_store(q, reinterpret_cast<v2*>(p + k));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < N; ++k) __builtin_nontemporal_store(v[k], p + k);
+  }
+}
+
+// sum of four doubles over a 256-thread block, one LDS round; sm is used by
+// tile parity so that consecutive tiles need one barrier each
+__device__ inline void block_sum4(double (&s)[4], double (*sm)[4], int parity) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j] += __shfl_down(s[j], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* buf = sm[parity * (kTB / 64) + wave];
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) buf[j] = s[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double a = 0.0;
+      for (int w = 0; w < kTB / 64; ++w) a += sm[parity * (kTB / 64) + w][j];
+      s[j] = a;
+    }
+  }
+}
+
+// (A^T y)_k at a pixel from y(p) (t=0: y0, t=1: y1), y0 of the pixel above and y1 of the pixel to the left
+template <typename T, int C>
+__device__ inline void adjoint_px(const T* y0, const T* y1, const T* yup0, const T* yleft1, int r, int c, int H, int W,
+                                  const Cfg& cfg, T* out) {
+  const T ih0 = (T)cfg.ih0, ih1 = (T)cfg.ih1;
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    T d = T(0);
+    if (r < H - 1) d -= y0[k] * ih0;
+    if (r > 0) d += yup0[k] * ih0;
+    if (c < W - 1) d -= y1[k] * ih1;
+    if (c > 0) d += yleft1[k] * ih1;
+    out[k] = (T)cfg.w[k] * d;
+  }
+}
+
+// gradient at a pixel from its value, the value below and the value to the right
+template <typename T, int C>
+__device__ inline void grad_px(const T* v, const T* vdown, const T* vright, int r, int c, int H, int W, const Cfg& cfg,
+                               T* g) {
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    g[k] = (r < H - 1) ? (T)(cfg.w[k] * cfg.ih0) * (vdown[k] - v[k]) : T(0);
+    g[C + k] = (c < W - 1) ? (T)(cfg.w[k] * cfg.ih1) * (vright[k] - v[k]) : T(0);
+  }
+}
+
+// y+ = proj_{|.| <= tvw}(y + sigma * grad(xbar)) at one pixel; g receives grad(xbar)
+template <typename T, int C>
+__device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W, T sigma,
+                                T tvw, const Cfg& cfg, T* yp, T* g) {
+  grad_px<T, C>(xb, xbd, xbr, r, c, H, W, cfg, g);
+  T n2 = T(0);
+#pragma unroll
+  for (int e = 0; e < 2 * C; ++e) {
+    T s = sigma;
+    if (cfg.steps == OPY_TV_STEPS_PRECOND)
+      s = (e < C) ? precond_sigma<T>(r < H - 1, e, cfg.ih0, cfg) : precond_sigma<T>(c < W - 1, e - C, cfg.ih1, cfg);
+    yp[e] = y[e] + s * g[e];
+    n2 += yp[e] * yp[e];
+  }
+  if (n2 > tvw * tvw) {
+    const T sc = tvw / sqrt(n2);
+#pragma unroll
+    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+  }
+}
+
+// ---- one pass per iteration -------------------------------------------------
+// Cross-lane moves.  DPP wave shifts (one VALU instruction): shr1 gives lane
+// L the value of lane L-1 and lane 0 `edge`; shl1 gives lane L the value of
+// lane L+1 and lane 63 `edge`.  readlane broadcasts lane l's value.
+__device__ inline int dpp_shr1_i(int edge, int v) { return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xF, 0xF, false); }
+__device__ inline int dpp_shl1_i(int edge, int v) { return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xF, 0xF, false); }
+__device__ inline float dpp_shr1(float edge, float v) { return __int_as_float(dpp_shr1_i(__float_as_int(edge), __float_as_int(v))); }
+__device__ inline float dpp_shl1(float edge, float v) { return __int_as_float(dpp_shl1_i(__float_as_int(edge), __float_as_int(v))); }
+__device__ inline double dpp_shr1(double edge, double v) {
+  return __hiloint2double(dpp_shr1_i(__double2hiint(edge), __double2hiint(v)),
+                          dpp_shr1_i(__double2loint(edge), __double2loint(v)));
+}
+__device__ inline double dpp_shl1(double edge, double v) {
+  return __hiloint2double(dpp_shl1_i(__double2hiint(edge), __double2hiint(v)),
+                          dpp_shl1_i(__double2loint(edge), __double2loint(v)));
+}
+__device__ inline float readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline double readlane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// ---- host wrappers ---------------------------------------------------------
+#define KF(...) reinterpret_cast<const void*>(&__VA_ARGS__)
+
+// Blocks of `kernel` resident at once on the current device (occupancy x
+// CUs), cached per kernel and device: the persistent loops need no more.
+int resident_blocks(const void* kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(kernel, dev);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTB, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (dev < 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  const int g = per_cu * cus;
+  cache[key] = g;
+  return g;
+}
+
+// grid > 0: as given; grid == 0: the resident block count of the kernel
+inline int launch_grid(const void* kernel, int grid) { return grid > 0 ? grid : resident_blocks(kernel); }
+
+int check_cfg(const opy_tv_cfg_t* c) {
+  if (!c) return fail(kInvalid, "tv: null config");
+  if (c->C < 1 || c->C > 4) return fail(kInvalid, "tv: C=%d outside [1,4]", c->C);
+  if (c->tw < 1 || c->tw > kTB || kTB % c->tw) return fail(kInvalid, "tv: tile width %d must divide %d", c->tw, kTB);
+  if (c->rpt != 1 && c->rpt != 2 && c->rpt != 4 && c->rpt != 6 && c->rpt != 8 && c->rpt != 12 && c->rpt != 16)
+    return fail(kInvalid, "tv: rows per thread %d not in {1,2,4,6,8,12,16}", c->rpt);
+  if (c->steps != OPY_TV_STEPS_SCALAR && c->steps != OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv: bad step mode");
+  if (!(c->h0 > 0) || !(c->h1 > 0)) return fail(kInvalid, "tv: grid spacing must be positive");
+  return kOk;
+}
+
+// dispatch a kernel template over C (1..4) and RPT (1,2,4,6,8,12,16)
+#define OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, CC)       \
+  switch (cfg->rpt) {                                \
+    case 1: KERNEL_EXPR_C_R(CC, 1); break;           \
+    case 2: KERNEL_EXPR_C_R(CC, 2); break;           \
+    case 4: KERNEL_EXPR_C_R(CC, 4); break;           \
+    case 6: KERNEL_EXPR_C_R(CC, 6); break;           \
+    case 8: KERNEL_EXPR_C_R(CC, 8); break;           \
+    case 12: KERNEL_EXPR_C_R(CC, 12); break;         \
+    default: KERNEL_EXPR_C_R(CC, 16); break;         \
+  }
+#define OPY_TV_DISPATCH(KERNEL_EXPR_C_R)                     \
+  switch (cfg->C) {                                          \
+    case 1: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 1); break;    \
+    case 2: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 2); break;    \
+    case 3: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 3); break;    \
+    default: OPY_TV_DISPATCH_C(KERNEL_EXPR_C_R, 4); break;   \
+  }
+
+}  // namespace
+}  // namespace opy
