@@ -5,12 +5,14 @@ for the batched fused PDHG on the learned-warm-start workload: TV-L2
 denoising of 1024x1024 images from the rectangle generator, lambda
 log-uniform in [1, 20], 256 instances per batch.  A step = solving one
 resident batch from its start point (cold, or the network's prediction with
---warm) until every instance is certified or the iteration cap.
+--warm) until every instance is certified or the iteration cap.  The batched
+solver runs the accelerated steps of Chambolle & Pock (2011, Algorithm 2) on
+these strongly convex instances (--step-rule adaptive: the reference's rule).
 
 Printed next to it:
   * the plain single-instance GPU PDHG (the drop-in generic kernel chain of
-    PDHG.solve, cold start, same step rule and cap) on instances spread over
-    the lambda range of the same batch;
+    PDHG.solve, cold start, the reference's adaptive steps, same cap) on
+    instances spread over the lambda range of the same batch;
   * the roofline of the dominant kernel (the single-pass iteration kernel
     with adaptive steps; the dual pass with --passes 2): algorithmic bytes of
     a full-batch launch over the average duration of eager full-batch
@@ -56,17 +58,17 @@ def build_batch(args):
     return bat, specs
 
 
-def roofline_phase(bat, reps):
-    """Eager full-batch launches of the iteration kernel(s) with adaptive
-    steps (every instance running), timed with HIP events on the stream they
-    are launched on; one launch of the library's copy kernel over a buffer of
-    known size for counter scaling."""
+def roofline_phase(bat, reps, rule):
+    """Eager full-batch launches of the iteration kernel(s) with the bench's
+    step rule (every instance running), timed with HIP events on the stream
+    they are launched on; one launch of the library's copy kernel over a
+    buffer of known size for counter scaling."""
     import ctypes
     import torch
     from opymize_amd import _hip
-    steps = "adaptive"
-    cfg = bat._cfg(steps)
-    bat._reset_state(steps, 0.124, 1.0, 1e-5, 1e-5, True)
+    cfg = bat._cfg(rule)
+    adapt = int(rule == "adaptive")
+    bat._reset_state(rule, 0.124, 1.0, 1e-5, 1e-5, True)
     P, sfx = _hip.ptr, bat._sfx()
     stream = torch.cuda.current_stream()
     s = _hip.stream()
@@ -75,10 +77,10 @@ def roofline_phase(bat, reps):
         if name == "iter":
             bat._call("opy_tv_iter" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.f),
                       P(bat.insts), P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact),
-                      ctypes.byref(cfg), 1, P(bat.part_res), bat.grid, s)
+                      ctypes.byref(cfg), adapt, P(bat.part_res), bat.grid, s)
         elif name == "dual":
             bat._call("opy_tv_dual" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.insts),
-                      P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact), ctypes.byref(cfg), 1,
+                      P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact), ctypes.byref(cfg), adapt,
                       P(bat.part_res), bat.grid, s)
         else:
             bat._call("opy_tv_primal" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.f), P(bat.insts),
@@ -104,12 +106,12 @@ def roofline_phase(bat, reps):
     return out
 
 
-def measured_traffic(npx, tile, passes):
+def measured_traffic(npx, tile, passes, rule):
     """HBM bytes per full-batch launch of the dominant kernel from the
     committed counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it
     reads for a copy of known size), when they were taken on this workload;
     else None."""
-    name = "tv_iter_kernel" if passes == 1 else "tv_dual_kernel"
+    name = ("tv_iter_kernel" if passes == 1 else "tv_dual_kernel") + ("_accel" if rule == "accel" else "")
     path = os.path.join(ROOT, "profiles", "%s_tvl2_256x1024_counters.json" % name)
     if not os.path.exists(path):
         return None
@@ -153,10 +155,10 @@ def plain_single_instance(bat, specs, n, max_iters):
             "seconds": t_tot, "lams": [round(specs[b]["lam"], 3) for b in pick]}
 
 
-def cpu_baseline(bat, specs, budget_s, gpu_iters):
-    """Oracle (float64 numpy/scipy, one thread) on instance 0: seconds per
-    iteration over a bounded sample, scaled by the iterations the GPU needed
-    for that instance."""
+def cpu_baseline(bat, specs, budget_s, gpu_iters, rule):
+    """Oracle (float64 numpy/scipy, one thread, same step rule) on instance 0:
+    seconds per iteration over a bounded sample, scaled by the iterations the
+    GPU needed for that instance."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     from oracle import pdhg as opd
     f = bat.unpack(bat.f, "x")[0].reshape(-1, 1)
@@ -164,18 +166,18 @@ def cpu_baseline(bat, specs, budget_s, gpu_iters):
     prob = opd.tv_problem(f, (H, W), specs[0]["lam"])
     n_it = 5
     t = time.perf_counter()
-    opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
+    opd.solve(prob, steps=rule, step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
     dt = time.perf_counter() - t
     while dt < budget_s and n_it < 400:
         n_it = min(400, max(n_it * 2, int(n_it * budget_s / max(dt, 1e-3))))
         t = time.perf_counter()
-        opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
+        opd.solve(prob, steps=rule, step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
         dt = time.perf_counter() - t
     per_it = dt / n_it
     return {"value": 1.0 / (per_it * gpu_iters), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle (float64 numpy/scipy, adaptive steps, 1 thread) on instance 0 (%dx%d): %d iterations "
+            "sample": "oracle (float64 numpy/scipy, %s steps, 1 thread) on instance 0 (%dx%d): %d iterations "
                       "in %.1fs, scaled to the %d iterations the GPU needed for that instance"
-                      % (H, W, n_it, dt, gpu_iters)}
+                      % (rule, H, W, n_it, dt, gpu_iters)}
 
 
 def main():
@@ -191,6 +193,8 @@ def main():
     ap.add_argument("--tile", type=int, nargs=2, default=[64, 8])
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
     ap.add_argument("--warm", action="store_true", help="start from the trained network's prediction")
+    ap.add_argument("--step-rule", default="accel", choices=["accel", "adaptive"],
+                    help="step rule of the batched solver")
     ap.add_argument("--plain", type=int, default=6, help="instances for the plain single-instance baseline")
     ap.add_argument("--reps", type=int, default=20, help="launches per pass in the roofline phase")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -211,7 +215,7 @@ def main():
     lams = [s["lam"] for s in specs]
 
     if args.roofline_only:
-        kt = roofline_phase(bat, args.reps)
+        kt = roofline_phase(bat, args.reps, args.step_rule)
         log("roofline phase: %s" % ", ".join("%s %.4f ms" % kv for kv in kt.items()))
         return
 
@@ -225,7 +229,7 @@ def main():
             warm.predict_into(bat, lams)
         else:
             bat.set_start(None, None)
-        return bat.solve(steps="adaptive", step_bound=0.124, max_iters=args.max_iters,
+        return bat.solve(steps=args.step_rule, step_bound=0.124, max_iters=args.max_iters,
                          block_iters=args.block_iters, tol=1e-5, certify=True, fallback=warm is not None)
 
     for _ in range(args.warmup):
@@ -245,13 +249,14 @@ def main():
     log("batched: %d/%d certified in %.2fs, iterations min/median/max %d/%d/%d"
         % (certified, args.steps * args.batch, elapsed, iters.min(), np.median(iters), iters.max()))
 
-    kt = roofline_phase(bat, args.reps)
+    kt = roofline_phase(bat, args.reps, args.step_rule)
+    variant = "adaptive" if args.step_rule == "adaptive" else "accelerated steps"
     if bat.passes == 1:
         kname, kms, kbytes = "iter", kt["iter"], ITER_BYTES_PER_PX
-        kdesc = "tv_iter_kernel<float,1,%d,adaptive> (one pass per iteration)" % args.tile[1]
+        kdesc = "tv_iter_kernel<float,1,%d>, %s (one pass per iteration)" % (args.tile[1], variant)
     else:
         kname, kms, kbytes = "dual", kt["dual"], DUAL_BYTES_PER_PX
-        kdesc = "tv_dual_kernel<float,1,%d,adaptive> (second of two passes)" % args.tile[1]
+        kdesc = "tv_dual_kernel<float,1,%d>, %s (second of two passes)" % (args.tile[1], variant)
     achieved = kbytes * npx / (kms * 1e-3) / 1e9
     log("roofline phase: %s -- %s %.0f GB/s algorithmic" % (
         ", ".join("%s %.3f ms" % kv for kv in kt.items()), kname, achieved))
@@ -263,7 +268,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bat, specs, args.cpu_budget, int(iters[0]))
+        cpu = cpu_baseline(bat, specs, args.cpu_budget, int(iters[0]), args.step_rule)
         log("cpu oracle: %s" % cpu["sample"])
 
     line = {
@@ -272,8 +277,11 @@ def main():
         "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "TV-L2 denoising, %d instances of %dx%d per batch, rectangle generator, lambda "
-                               "log-uniform [1,20], adaptive steps, rel gap 1e-5 FP64-certified, cap %d iterations"
+                               "log-uniform [1,20], rel gap 1e-5 FP64-certified, cap %d iterations"
                                % (args.batch, args.size, args.size, args.max_iters),
+                   "step_rule": {"accel": "accelerated (Chambolle-Pock Alg. 2, gamma = lambda)",
+                                 "adaptive": "adaptive (reference rule)"}[args.step_rule],
+                   "plain_step_rule": "adaptive (reference rule)",
                    "warm_start": "cnn (6x3x3 conv, 32 ch, bf16 MFMA) with cold-start fallback" if warm else "none",
                    "tile": list(args.tile), "block_iters": args.block_iters,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
@@ -282,7 +290,7 @@ def main():
                    "plain_instances": len(plain["instances"]) if plain else 0,
                    "ratio_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(npx, args.tile, args.passes),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(npx, args.tile, args.passes, args.step_rule),
                      "kernel": "%s: %d B/pixel algorithmic x %d pixels per launch" % (kdesc, kbytes, npx),
                      "avg_ms": kms, "launch_ms": kt},
         "cpu_baseline": cpu,

@@ -258,6 +258,11 @@ int opy_fn_eval_f64(int kind, const double* x, int64_t npts, int width, const op
 #define OPY_TV_DATA_L1 1      /* G = lam*|x - f|_1      (TV-L1 data term, new) */
 #define OPY_TV_STEPS_SCALAR 0 /* tau, sigma from the instance's opy_adapt_state_t (constant or adaptive) */
 #define OPY_TV_STEPS_PRECOND 1/* diagonal preconditioners of the stencil (reference pdhg.py:189-200) */
+/* accelerated steps for the lam-strongly convex SSD term (Chambolle & Pock 2011,
+ * Algorithm 2, no reference counterpart): per instance theta = 1/sqrt(1 + 2 lam tau)
+ * (1 for TV-L1), xbar = x+ + theta (x+ - x), dual step sigma/theta; opy_tv_accel
+ * then sets tau <- theta tau, sigma <- sigma/theta */
+#define OPY_TV_STEPS_ACCEL 2
 
 typedef struct {
   int64_t off;    /* element offset of the instance's x and f (pixels*C); y starts at 2*off */
@@ -320,6 +325,9 @@ int opy_tv_iter_f64(const double* x_in, double* x_out, const double* y_in, doubl
 /* adaptive step rule per running instance from the residual partials (reference pdhg.py:155-183) */
 int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks,
                  const double* partials, int B, int C, const opy_adapt_params_t* params, void* stream);
+/* accelerated step update of the running instances after an iteration (OPY_TV_STEPS_ACCEL) */
+int opy_tv_accel(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks, int B,
+                 void* stream);
 /* FP64 gap partials per tile (reference pdhg.py:23-44) */
 int opy_tv_gap_f32(const float* x, const float* y, const float* f, const opy_tv_inst_t* insts,
                    const int32_t* active_inst, const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg,

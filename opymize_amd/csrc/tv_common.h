@@ -125,6 +125,13 @@ __device__ inline T precond_sigma(bool inner, int k, double ih, const Cfg& cfg) 
   return (T)(s > kEps ? 1.0 / s : s);
 }
 
+// Over-relaxation of the iteration: theta = 1/sqrt(1 + 2 lam tau) with
+// accelerated steps on the lam-strongly convex SSD term (Chambolle & Pock
+// 2011, Algorithm 2), else 1 (the reference's theta, pdhg.py:145-153).
+__device__ inline double accel_theta(int steps, const opy_tv_inst_t& I, double tau) {
+  return (steps == OPY_TV_STEPS_ACCEL && I.data == OPY_TV_DATA_L2) ? 1.0 / sqrt(1.0 + 2.0 * I.lam * tau) : 1.0;
+}
+
 template <typename T>
 __device__ inline T prox_data(T v, T f, double tl, int data) {
   if (data == OPY_TV_DATA_L1) {
@@ -314,7 +321,8 @@ int check_cfg(const opy_tv_cfg_t* c) {
   if (c->tw < 1 || c->tw > kTB || kTB % c->tw) return fail(kInvalid, "tv: tile width %d must divide %d", c->tw, kTB);
   if (c->rpt != 1 && c->rpt != 2 && c->rpt != 4 && c->rpt != 6 && c->rpt != 8 && c->rpt != 12 && c->rpt != 16)
     return fail(kInvalid, "tv: rows per thread %d not in {1,2,4,6,8,12,16}", c->rpt);
-  if (c->steps != OPY_TV_STEPS_SCALAR && c->steps != OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv: bad step mode");
+  if (c->steps != OPY_TV_STEPS_SCALAR && c->steps != OPY_TV_STEPS_PRECOND && c->steps != OPY_TV_STEPS_ACCEL)
+    return fail(kInvalid, "tv: bad step mode");
   if (!(c->h0 > 0) || !(c->h1 > 0)) return fail(kInvalid, "tv: grid spacing must be positive");
   return kOk;
 }

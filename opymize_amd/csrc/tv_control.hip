@@ -51,6 +51,18 @@ __global__ void __launch_bounds__(kTB) tv_adapt_kernel(const opy_tv_inst_t* __re
   }
 }
 
+// ---- accelerated step update (Chambolle & Pock 2011, Algorithm 2) -----------
+// tau <- theta tau, sigma <- sigma / theta with the theta the iteration used
+__global__ void __launch_bounds__(kTB) tv_accel_kernel(const opy_tv_inst_t* __restrict__ insts,
+                                                       opy_adapt_state_t* __restrict__ st,
+                                                       const opy_tv_check_t* __restrict__ chk, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || chk[b].status != 0) return;
+  const double th = accel_theta(OPY_TV_STEPS_ACCEL, insts[b], st[b].tau);
+  st[b].tau *= th;
+  st[b].sigma /= th;
+}
+
 // ---- primal-dual gap (reference pdhg.py:23-44) ------------------------------
 // per tile partials: [P_data, P_tv, D_lin = sum z f, D_quad = sum z^2, max |y_p|, max |z|] with z = A^T y
 template <typename T, int C, int RPT>
@@ -280,6 +292,12 @@ int gap_impl(const T* x, const T* y, const T* f, const opy_tv_inst_t* insts, con
 
 using namespace opy;
 
+OPY_API int opy_tv_accel(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk, int B,
+                         void* s) {
+  if (!insts || !st || !chk || B < 1) return fail(kInvalid, "tv_accel: bad arguments");
+  hipLaunchKernelGGL(tv_accel_kernel, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, as_stream(s), insts, st, chk, B);
+  return check_launch("tv_accel");
+}
 OPY_API int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk,
                          const double* partials, int B, int C, const opy_adapt_params_t* p, void* s) {
   if (!insts || !st || !chk || !partials || !p || B < 1) return fail(kInvalid, "tv_adapt: bad arguments");

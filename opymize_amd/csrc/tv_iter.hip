@@ -44,7 +44,9 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
     const opy_tv_inst_t I = insts[tr.b];
     const int H = I.H, W = I.W;
     const double tau_s = st[tr.b].tau;
-    const T sigma = (T)st[tr.b].sigma;
+    const double th_d = accel_theta(cfg.steps, I, tau_s);
+    const T theta = (T)th_d;                      // 1 unless accelerated
+    const T sigma = (T)(st[tr.b].sigma / th_d);
     const T tvw = (T)I.tvw;
     const int c0 = tr.c0;
     const int c = c0 + lane;
@@ -142,13 +144,13 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
       for (int k = 0; k < C; ++k) {
         const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(rx, cr, H, W, k, cfg) : (T)tau_s;
         const T p = prox_data<T>(xrh[k] - tau * ah[k], frh[k], (double)tau * I.lam, I.data);
-        xbh[k] = T(2) * p - xrh[k];
+        xbh[k] = (T(1) + theta) * p - theta * xrh[k];
         xdh[k] = xrh[k] - p;
       }
     }
     // xbar = 2x+ - x and xd = x - x+ (and their right neighbours) are formed
     // where they are used, which keeps fewer values live
-    auto xbar = [&](int j, int k) { return T(2) * xp[j][k] - xv[j][k]; };
+    auto xbar = [&](int j, int k) { return (T(1) + theta) * xp[j][k] - theta * xv[j][k]; };
     // ---- left halo (adaptive), transposed: lane k forms y+ at (y+ row k, cl), t=1 part
     T yplh[C];
 #pragma unroll
@@ -163,7 +165,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
 #pragma unroll
       for (int k = 0; k < C; ++k) {
         const T p = prox_data<T>(xlh[k] - (T)tau_s * ah[k], flh[k], tau_s * I.lam, I.data);
-        xbl[k] = T(2) * p - xlh[k];
+        xbl[k] = (T(1) + theta) * p - theta * xlh[k];
       }
 #pragma unroll
       for (int k = 0; k < C; ++k) {
@@ -268,7 +270,8 @@ int iter_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_i
     return fail(kInvalid, "tv_iter: null pointer or negative grid");
   if (xi == xo || yi == yo) return fail(kInvalid, "tv_iter: ping-pong buffers must differ");
   if (adapt && !partials) return fail(kInvalid, "tv_iter: adaptive steps need a partials buffer");
-  if (adapt && cfg->steps == OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv_iter: adaptive and precond exclude each other");
+  if (adapt && cfg->steps != OPY_TV_STEPS_SCALAR)
+    return fail(kInvalid, "tv_iter: adaptive steps exclude precond and accelerated steps");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
   if (adapt) {

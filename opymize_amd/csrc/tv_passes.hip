@@ -111,7 +111,9 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
     const TileRef tr = locate(g, active_inst, act_off, nact, insts, tw, th);
     const opy_tv_inst_t I = insts[tr.b];
     const int H = I.H, W = I.W;
-    const T sigma = (T)st[tr.b].sigma;
+    const double th_d = accel_theta(cfg.steps, I, st[tr.b].tau);
+    const T theta = (T)th_d;                      // 1 unless accelerated
+    const T sigma = (T)(st[tr.b].sigma / th_d);
     const T tau = (T)st[tr.b].tau;
     const T tvw = (T)I.tvw;
     const int c = tr.c0 + tx;
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
       for (int k = 0; k < C; ++k) {
         const T a = ax[j][k], b = bx[j][k];
         ax[j][k] = a - b;
-        bx[j][k] = T(2) * b - a;
+        bx[j][k] = (T(1) + theta) * b - theta * a;
       }
 #pragma unroll
     for (int j = j0; j <= RPT; ++j)
@@ -194,11 +196,11 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
         const T a = axr[j][k], b = bxr[j][k];
         const T sd = shfl_down1(ax[j][k]), sb = shfl_down1(bx[j][k]);
         axr[j][k] = rhalo ? a - b : sd;
-        bxr[j][k] = rhalo ? T(2) * b - a : sb;
+        bxr[j][k] = rhalo ? (T(1) + theta) * b - theta * a : sb;
         if (kAdapt) {
           const T al = axl[j][k], bl = bxl[j][k];
           axl[j][k] = al - bl;
-          bxl[j][k] = T(2) * bl - al;
+          bxl[j][k] = (T(1) + theta) * bl - theta * al;
         }
       }
     double s[4] = {0.0, 0.0, 0.0, 0.0};
@@ -304,7 +306,8 @@ int dual_impl(const T* xo, const T* xn, const T* yi, T* yo, const opy_tv_inst_t*
     return fail(kInvalid, "tv_dual: null pointer or negative grid");
   if (yi == yo || xo == xn) return fail(kInvalid, "tv_dual: ping-pong buffers must differ");
   if (adapt && !partials) return fail(kInvalid, "tv_dual: adaptive steps need a partials buffer");
-  if (adapt && cfg->steps == OPY_TV_STEPS_PRECOND) return fail(kInvalid, "tv_dual: adaptive and precond exclude each other");
+  if (adapt && cfg->steps != OPY_TV_STEPS_SCALAR)
+    return fail(kInvalid, "tv_dual: adaptive steps exclude precond and accelerated steps");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
   if (adapt) {

@@ -22,6 +22,10 @@ import torch
 from opymize_amd import _hip
 
 _DATA = {"l2": 0, "l1": 1}
+# step rules: the reference's constant / adaptive (pdhg.py:145-183) and
+# preconditioned (pdhg.py:189-209) steps, and the accelerated steps of
+# Chambolle & Pock (2011, Algorithm 2) for the strongly convex TV-L2 instances
+STEP_MODES = {"const": 0, "adaptive": 0, "precond": 1, "accel": 2}
 ALIGN = 64  # elements; every instance starts on a 256-byte boundary (FP32)
 
 
@@ -55,6 +59,7 @@ _SIGS = {
 }
 _PLAIN = {
     "opy_tv_adapt": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+    "opy_tv_accel": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_check": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_compact": [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int,
@@ -175,7 +180,7 @@ class TVBatch(object):
     def _cfg(self, steps):
         c = TVCfg()
         c.C, c.tw, c.rpt = self.C, self.tw, self.rpt
-        c.steps = 1 if steps == "precond" else 0
+        c.steps = STEP_MODES[steps]
         c.h0, c.h1 = self.h
         for k in range(4):
             c.w[k] = float(self.weights[k]) if k < self.C else 1.0
@@ -257,6 +262,8 @@ class TVBatch(object):
         if adapt:
             self._call("opy_tv_adapt", P(self.insts), P(self.state), P(self.checks), P(self.part_res), self.B,
                        self.C, ctypes.byref(params), s)
+        elif cfg.steps == STEP_MODES["accel"]:
+            self._call("opy_tv_accel", P(self.insts), P(self.state), P(self.checks), self.B, s)
 
     def _gap_check(self, cfg, tol, tol_infeas, certify, iters_in_block):
         P = _hip.ptr
@@ -329,6 +336,8 @@ class TVBatch(object):
         fallback=True (for warm starts): the gap of the start point is
         recorded, and an instance whose gap has not improved at the first
         test (or is not finite) restarts cold from x = y = 0."""
+        if steps not in STEP_MODES:
+            raise ValueError("steps must be one of %s" % sorted(STEP_MODES))
         tol_infeas = tol if tol_infeas is None else tol_infeas
         if fallback:
             check_start = True

@@ -10,11 +10,12 @@ class Problem(object):
     """min_x G(x) + F(Ax) given by its sparse A, the two prox maps and the
     four (value, infeasibility) evaluations used by the gap."""
 
-    def __init__(self, A, gprox, fconjprox, G, F, Gc, Fc):
+    def __init__(self, A, gprox, fconjprox, G, F, Gc, Fc, gamma=0.0):
         self.A = A.tocsr()
         self.AT = self.A.T.tocsr()
         self.gprox, self.fconjprox = gprox, fconjprox
         self.G, self.F, self.Gc, self.Fc = G, F, Gc, Fc
+        self.gamma = gamma  # strong convexity modulus of G (accelerated steps)
 
 
 def tv_problem(f, dims, lam, data="l2", tv_lbd=1.0, matrixnorm="frobenius", h=None, w=None):
@@ -43,7 +44,7 @@ def tv_problem(f, dims, lam, data="l2", tv_lbd=1.0, matrixnorm="frobenius", h=No
     ballnorm = "spectral" if matrixnorm == "nuclear" else "frobenius"
     F = lambda z: (tv_lbd * prox.block_norms(z, N, C, D, sumnorm).sum(), 0.0)
     Fc = lambda y: (0.0, float(max(0.0, (prox.block_norms(y, N, C, D, ballnorm) - tv_lbd).max())))
-    return Problem(A, gprox, fconjprox, G, F, Gc, Fc)
+    return Problem(A, gprox, fconjprox, G, F, Gc, Fc, gamma=float(lamv.min()) if data == "l2" else 0.0)
 
 
 def truncate(x, n):
@@ -126,7 +127,7 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
         if steps == "adaptive":
             eta, Delta, alpha = 0.95, 1.5, 0.5
             tau = sigma = np.sqrt(bnd)
-        else:
+        else:  # constant, and the start of accelerated steps
             sigma = np.sqrt(bnd / step_factor)
             tau = bnd / sigma
     hist = []
@@ -136,6 +137,12 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
         xn = x - tau * xgrad
         xn = prob.gprox(xn, tau)
         ygn = A.dot(xn)
+        if steps == "accel":
+            # Chambolle & Pock (2011), Algorithm 2 in primal-first order:
+            # theta_k = 1/sqrt(1 + 2 gamma tau_k), sigma_k+1 = sigma_k/theta_k,
+            # xbar = x_k+1 + theta_k (x_k+1 - x_k), tau_k+1 = theta_k tau_k
+            theta = 1.0 / np.sqrt(1.0 + 2.0 * prob.gamma * tau)
+            sigma = sigma / theta
         ybar = (1 + theta) * ygn - theta * ygrad
         yn = y + sigma * ybar
         yn = prob.fconjprox(yn, sigma)
@@ -155,6 +162,8 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
                 tau, sigma, alpha = tau / (1 - alpha), sigma * (1 - alpha), alpha * eta
             if resp < scale * resd / Delta:
                 tau, sigma, alpha = tau * (1 - alpha), sigma / (1 - alpha), alpha * eta
+        if steps == "accel":
+            tau = theta * tau
         x, xgrad, y, ygrad = xn, xgn, yn, ygn
         if it % granularity == 0:
             info["iter"] = it
@@ -169,4 +178,6 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
     info["iter"] = it
     if steps == "adaptive":
         info.update(tau=tau, sigma=sigma, alpha=alpha)
+    elif steps == "accel":
+        info.update(tau=tau, sigma=sigma)
     return x, y, info, hist

@@ -152,3 +152,25 @@ def test_autotune_picks_a_candidate_and_keeps_results(gpu, passes):
         xo, _, _, _ = opd.solve(prob, steps="adaptive", step_bound=0.124, term_maxiter=k, granularity=k,
                                 term_pd_gap=1e-30)
         assert np.abs(xs[b] - xo).max() < 1e-9, b
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("passes,tile", [(1, (64, 8)), (1, (64, 4)), (2, (64, 2))])
+def test_accelerated_steps_match_oracle(gpu, precision, passes, tile):
+    """steps="accel": per-instance theta, tau, sigma of Chambolle & Pock's
+    Algorithm 2 on the TV-L2 instances (the TV-L1 ones keep constant steps)."""
+    bat, imgs, lams, data = make_batch(precision, tile, passes)
+    k = 40
+    bat.solve(steps="accel", step_bound=0.124, max_iters=k, block_iters=20, tol=1e-30)
+    xs, ys = bat.solution()
+    st = bat.steps_state()
+    tol = 1e-9 if precision == "double" else 5e-4
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps="accel", step_bound=0.124, term_maxiter=k, granularity=k,
+                                  term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < tol, b
+        assert np.abs(ys[b] - yo).max() < tol, b
+        assert abs(st[b]["tau"] - io["tau"]) <= 1e-12 * io["tau"], b
+        assert abs(st[b]["sigma"] - io["sigma"]) <= 1e-12 * io["sigma"], b
