@@ -13,10 +13,11 @@ Printed next to it:
   * the plain single-instance GPU PDHG (the drop-in generic kernel chain of
     PDHG.solve, cold start, the reference's adaptive steps, same cap) on
     instances spread over the lambda range of the same batch;
-  * the roofline of the dominant kernel (the single-pass iteration kernel
-    with adaptive steps; the dual pass with --passes 2): algorithmic bytes of
-    a full-batch launch over the average duration of eager full-batch
-    launches, timed with HIP events on the launch stream;
+  * the roofline of the dominant kernel (by default the kernel that runs K
+    iterations per launch on LDS-staged tiles; the single-pass iteration
+    kernel with --multi 1 or adaptive steps; the dual pass with --passes 2):
+    algorithmic bytes of a full-batch launch over the average duration of
+    eager full-batch launches, timed with HIP events on the launch stream;
   * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--warm]
@@ -35,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ITER_BYTES_PER_PX = 28.0    # single pass, C=1 FP32: reads x_k, f (4+4 B), y_k (8 B); writes x_k+1 (4 B), y_k+1 (8 B)
+MULTI_BYTES_PER_PX = 28.0   # K iterations per launch: reads x_k, f, y_k and writes x_k+K, y_k+K once
 DUAL_BYTES_PER_PX = 24.0    # two passes: the dual pass reads x_k, x_k+1 (4+4 B), y_k (8 B); writes y_k+1 (8 B)
 PRIMAL_BYTES_PER_PX = 20.0  # two passes: the primal pass reads x_k, f (4+4 B), y_k (8 B); writes x_k+1 (4 B)
 
@@ -52,7 +54,8 @@ def build_batch(args):
     imgs = synth.images(specs)
     log("generated %d images of %dx%d in %.1fs" % (len(imgs), args.size, args.size, time.time() - t))
     bat = TVBatch(imgs, [s["lam"] for s in specs], "l2", precision="float", tile=tuple(args.tile),
-                  passes=args.passes)
+                  passes=args.passes,
+                  multi=args.multi if (args.step_rule != "adaptive" and args.passes == 1) else 1)
     del imgs
     torch.cuda.synchronize()
     return bat, specs
@@ -69,12 +72,17 @@ def roofline_phase(bat, reps, rule):
     cfg = bat._cfg(rule)
     adapt = int(rule == "adaptive")
     bat._reset_state(rule, 0.124, 1.0, 1e-5, 1e-5, True)
+    bat.refresh_grid()  # every instance running: one block per tile of the batch
     P, sfx = _hip.ptr, bat._sfx()
     stream = torch.cuda.current_stream()
     s = _hip.stream()
 
     def launch(name):
-        if name == "iter":
+        if name == "multi":
+            bat._call("opy_tv_multi" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.f),
+                      P(bat.insts), P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact),
+                      ctypes.byref(cfg), bat.multi, bat.grid, s)
+        elif name == "iter":
             bat._call("opy_tv_iter" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.f),
                       P(bat.insts), P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact),
                       ctypes.byref(cfg), adapt, P(bat.part_res), bat.grid, s)
@@ -86,8 +94,12 @@ def roofline_phase(bat, reps, rule):
             bat._call("opy_tv_primal" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.f), P(bat.insts),
                       P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact), ctypes.byref(cfg), bat.grid, s)
 
-    out = {}
-    for name in (("iter",) if bat.passes == 1 else ("dual", "primal")):
+    out = {"grid": bat.grid}
+    if bat.passes == 2:
+        names = ("dual", "primal")
+    else:
+        names = ("multi",) if (bat.multi > 1 and rule != "adaptive") else ("iter",)
+    for name in names:
         launch(name)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -106,12 +118,18 @@ def roofline_phase(bat, reps, rule):
     return out
 
 
-def measured_traffic(npx, tile, passes, rule):
+def measured_traffic(npx, tile, passes, rule, multi):
     """HBM bytes per full-batch launch of the dominant kernel from the
     committed counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it
     reads for a copy of known size), when they were taken on this workload;
     else None."""
-    name = ("tv_iter_kernel" if passes == 1 else "tv_dual_kernel") + ("_accel" if rule == "accel" else "")
+    if passes == 2:
+        name = "tv_dual_kernel"
+    elif multi > 1:
+        name = "tv_multi_kernel_k%d" % multi
+    else:
+        name = "tv_iter_kernel"
+    name += "_accel" if rule == "accel" else ""
     path = os.path.join(ROOT, "profiles", "%s_tvl2_256x1024_counters.json" % name)
     if not os.path.exists(path):
         return None
@@ -189,8 +207,10 @@ def main():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--max-iters", type=int, default=10000)
-    ap.add_argument("--block-iters", type=int, default=50)
+    ap.add_argument("--block-iters", type=int, default=48)
     ap.add_argument("--tile", type=int, nargs=2, default=[64, 8])
+    ap.add_argument("--multi", type=int, default=3, choices=[1, 2, 3, 4],
+                    help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
     ap.add_argument("--warm", action="store_true", help="start from the trained network's prediction")
     ap.add_argument("--step-rule", default="accel", choices=["accel", "adaptive"],
@@ -216,7 +236,7 @@ def main():
 
     if args.roofline_only:
         kt = roofline_phase(bat, args.reps, args.step_rule)
-        log("roofline phase: %s" % ", ".join("%s %.4f ms" % kv for kv in kt.items()))
+        log("roofline phase: %s" % ", ".join("%s %s" % kv for kv in kt.items()))
         return
 
     warm = None
@@ -251,15 +271,21 @@ def main():
 
     kt = roofline_phase(bat, args.reps, args.step_rule)
     variant = "adaptive" if args.step_rule == "adaptive" else "accelerated steps"
-    if bat.passes == 1:
-        kname, kms, kbytes = "iter", kt["iter"], ITER_BYTES_PER_PX
-        kdesc = "tv_iter_kernel<float,1,%d>, %s (one pass per iteration)" % (args.tile[1], variant)
-    else:
+    k_launch = 1
+    if bat.passes == 2:
         kname, kms, kbytes = "dual", kt["dual"], DUAL_BYTES_PER_PX
         kdesc = "tv_dual_kernel<float,1,%d>, %s (second of two passes)" % (args.tile[1], variant)
+    elif "multi" in kt:
+        k_launch = bat.multi
+        kname, kms, kbytes = "multi", kt["multi"], MULTI_BYTES_PER_PX
+        kdesc = "tv_multi_kernel<float,1,%d,%d>, %s (%d iterations per launch on LDS-staged tiles)" % (
+            args.tile[1], k_launch, variant, k_launch)
+    else:
+        kname, kms, kbytes = "iter", kt["iter"], ITER_BYTES_PER_PX
+        kdesc = "tv_iter_kernel<float,1,%d>, %s (one pass per iteration)" % (args.tile[1], variant)
     achieved = kbytes * npx / (kms * 1e-3) / 1e9
     log("roofline phase: %s -- %s %.0f GB/s algorithmic" % (
-        ", ".join("%s %.3f ms" % kv for kv in kt.items()), kname, achieved))
+        ", ".join("%s %s" % kv for kv in kt.items()), kname, achieved))
 
     plain = plain_single_instance(bat, specs, args.plain, args.max_iters) if args.plain > 0 else None
     if plain:
@@ -283,16 +309,19 @@ def main():
                                  "adaptive": "adaptive (reference rule)"}[args.step_rule],
                    "plain_step_rule": "adaptive (reference rule)",
                    "warm_start": "cnn (6x3x3 conv, 32 ch, bf16 MFMA) with cold-start fallback" if warm else "none",
-                   "tile": list(args.tile), "block_iters": args.block_iters,
+                   "tile": list(args.tile), "block_iters": args.block_iters, "iterations_per_launch": k_launch,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
                    "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max()),
                    "plain_single_instance_solves_per_sec": plain["value"] if plain else None,
                    "plain_instances": len(plain["instances"]) if plain else 0,
                    "ratio_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(npx, args.tile, args.passes, args.step_rule),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch),
                      "kernel": "%s: %d B/pixel algorithmic x %d pixels per launch" % (kdesc, kbytes, npx),
-                     "avg_ms": kms, "launch_ms": kt},
+                     "avg_ms": kms, "ms_per_iteration": kms / k_launch,
+                     "launch_ms": {k: v for k, v in kt.items() if k != "grid"},
+                     "blocks_per_launch": kt["grid"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -291,9 +291,11 @@ typedef struct {
   int32_t iters;                 /* iterations run when last checked */
 } opy_tv_check_t;
 
-/* The fused kernels run a persistent grid over the tiles of the running
- * instances: `grid` > 0 launches that many 256-thread blocks, 0 launches the
- * number of blocks resident at once on the device for that kernel. */
+/* The fused kernels walk the tiles of the running instances with a grid
+ * stride: `grid` > 0 launches that many 256-thread blocks, 0 launches the
+ * number of blocks resident at once on the device for that kernel.  One block
+ * per running tile (act_off[nact], read by the caller) streams fastest; blocks
+ * beyond the running tiles return at once. */
 /* pass 1: x_out = prox_{tau G}(x_in - tau A^T y) on the tiles of the running instances */
 int opy_tv_primal_f32(const float* x_in, float* x_out, const float* y, const float* f, const opy_tv_inst_t* insts,
                       const opy_adapt_state_t* steps, const int32_t* active_inst, const int32_t* act_off,
@@ -322,6 +324,20 @@ int opy_tv_iter_f64(const double* x_in, double* x_out, const double* y_in, doubl
                     const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
                     const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
                     double* partials, int grid, void* stream);
+/* k in {2,3,4} whole iterations in one pass over HBM (cfg->tw = 64, cfg->rpt in {2,4,8}; no
+ * adaptive steps, whose rule needs every iteration's residuals): the same iterates as k
+ * opy_tv_iter launches with steps SCALAR / PRECOND / ACCEL (PDHG.iteration_step, pdhg.py:124-153,
+ * k times).  Each block loads its tile and k pixels around it into LDS and iterates there.  The
+ * step state is read, not advanced: with OPY_TV_STEPS_ACCEL call opy_tv_accel k times after.
+ * Fails with an error when the region of a tile (3*C values per pixel) exceeds 64 KiB of LDS. */
+int opy_tv_multi_f32(const float* x_in, float* x_out, const float* y_in, float* y_out, const float* f,
+                     const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                     const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int grid,
+                     void* stream);
+int opy_tv_multi_f64(const double* x_in, double* x_out, const double* y_in, double* y_out, const double* f,
+                     const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                     const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int grid,
+                     void* stream);
 /* adaptive step rule per running instance from the residual partials (reference pdhg.py:155-183) */
 int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks,
                  const double* partials, int B, int C, const opy_adapt_params_t* params, void* stream);

@@ -132,17 +132,75 @@ __device__ inline double accel_theta(int steps, const opy_tv_inst_t& I, double t
   return (steps == OPY_TV_STEPS_ACCEL && I.data == OPY_TV_DATA_L2) ? 1.0 / sqrt(1.0 + 2.0 * I.lam * tau) : 1.0;
 }
 
-template <typename T>
-__device__ inline T prox_data(T v, T f, double tl, int data) {
-  if (data == OPY_TV_DATA_L1) {
-    const T thr = (T)tl;
-    const T d = v - f;
-    return f + (d > thr ? d - thr : (d < -thr ? d + thr : T(0)));
+// x+ = prox_{tau G}(x - tau (A^T y)) for the data terms: SSD gives
+// (v + tl f) / (1 + tl) with tl = tau lam, formed as a * (v + b f) with
+// a = 1/(1 + tl) and b = tl rounded to T; L1 gives f + soft(v - f, tl).
+// The coefficients depend on tau only, so they are formed once per tile:
+// for the instance's scalar step, or with preconditioned steps for an
+// interior pixel (both neighbours present along both axes, the same tau for
+// all of them); other pixels of a preconditioned run form theirs (an FP64
+// division each) where they occur.
+// PC: preconditioned steps (a template parameter, so that the scalar-step
+// kernels carry none of their selects).
+template <typename T, int C, bool PC>
+struct PrimalStep {
+  T tau[C], a[C], b[C];
+  double lam;
+  int data;
+  __device__ PrimalStep(double tau_s, const opy_tv_inst_t& I, const Cfg& cfg) : lam(I.lam), data(I.data) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) coef(PC ? precond_tau<T>(1, 1, 3, 3, k, cfg) : (T)tau_s, tau[k], a[k], b[k]);
   }
-  const T a = (T)(1.0 / (1.0 + tl));
-  const T b = (T)tl;
-  return a * (v + b * f);
-}
+  __device__ void coef(T t, T& tk, T& ak, T& bk) const {
+    const double tl = (double)t * lam;
+    tk = t;
+    bk = (T)tl;
+    ak = data == OPY_TV_DATA_L1 ? T(1) : (T)(1.0 / (1.0 + tl));
+  }
+  // x+ at pixel (r, c), channel k, from x, (A^T y) and f there; L1: the
+  // data kind as a compile-time constant; M = false: the pixel is known to
+  // be interior
+  template <bool L1, bool M = true>
+  __device__ T apply(T x, T aty, T f, int r, int c, int H, int W, int k, const Cfg& cfg) const {
+    T t = tau[k], ak = a[k], bk = b[k];
+    if (PC && M && !(r > 0 && r < H - 1 && c > 0 && c < W - 1)) coef(precond_tau<T>(r, c, H, W, k, cfg), t, ak, bk);
+    const T v = x - t * aty;
+    if constexpr (L1) {
+      const T d = v - f;
+      return f + (d > bk ? d - bk : (d < -bk ? d + bk : T(0)));
+    } else {
+      return ak * (v + bk * f);
+    }
+  }
+  __device__ T operator()(T x, T aty, T f, int r, int c, int H, int W, int k, const Cfg& cfg) const {
+    return data == OPY_TV_DATA_L1 ? apply<true>(x, aty, f, r, c, H, W, k, cfg)
+                                  : apply<false>(x, aty, f, r, c, H, W, k, cfg);
+  }
+};
+
+// dual step per component e of y (e < C: along axis 0, else axis 1): sigma,
+// or the preconditioner of a pixel with a neighbour along that axis (0 at
+// the last row / column), formed once per tile
+template <typename T, int C, bool PC>
+struct DualStep {
+  T s[2 * C];
+  __device__ DualStep(T sigma, const Cfg& cfg) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      s[k] = PC ? precond_sigma<T>(true, k, cfg.ih0, cfg) : sigma;
+      s[C + k] = PC ? precond_sigma<T>(true, k, cfg.ih1, cfg) : sigma;
+    }
+  }
+  template <bool M = true>
+  __device__ T at(int e, int r, int c, int H, int W) const {
+    return (PC && M && !(e < C ? r < H - 1 : c < W - 1)) ? T(0) : s[e];
+  }
+};
+
+// tvw / |v| for |v|^2 = n2 > tvw^2 > 0: one hardware reciprocal square root
+// (v_rsq_f32, within 1 ulp) in FP32, the correctly rounded quotient in FP64
+__device__ inline float ball_scale(float tvw, float n2) { return tvw * __builtin_amdgcn_rsqf(n2); }
+__device__ inline double ball_scale(double tvw, double n2) { return tvw / sqrt(n2); }
 
 // N consecutive values from an address aligned to min(N, 4) * sizeof(T) for
 // even N (per-pixel vectors inside instance blocks whose offsets are
@@ -219,49 +277,53 @@ __device__ inline void block_sum4(double (&s)[4], double (*sm)[4], int parity) {
   }
 }
 
-// (A^T y)_k at a pixel from y(p) (t=0: y0, t=1: y1), y0 of the pixel above and y1 of the pixel to the left
-template <typename T, int C>
+// (A^T y)_k at a pixel from y(p) (t=0: y0, t=1: y1), y0 of the pixel above
+// and y1 of the pixel to the left.  M = false: the pixel is known to have
+// all four neighbours (the same operations without the boundary selects).
+template <typename T, int C, bool M = true>
 __device__ inline void adjoint_px(const T* y0, const T* y1, const T* yup0, const T* yleft1, int r, int c, int H, int W,
                                   const Cfg& cfg, T* out) {
   const T ih0 = (T)cfg.ih0, ih1 = (T)cfg.ih1;
 #pragma unroll
   for (int k = 0; k < C; ++k) {
     T d = T(0);
-    if (r < H - 1) d -= y0[k] * ih0;
-    if (r > 0) d += yup0[k] * ih0;
-    if (c < W - 1) d -= y1[k] * ih1;
-    if (c > 0) d += yleft1[k] * ih1;
+    if (!M || r < H - 1) d -= y0[k] * ih0;
+    if (!M || r > 0) d += yup0[k] * ih0;
+    if (!M || c < W - 1) d -= y1[k] * ih1;
+    if (!M || c > 0) d += yleft1[k] * ih1;
     out[k] = (T)cfg.w[k] * d;
   }
 }
 
 // gradient at a pixel from its value, the value below and the value to the right
-template <typename T, int C>
+template <typename T, int C, bool M = true>
 __device__ inline void grad_px(const T* v, const T* vdown, const T* vright, int r, int c, int H, int W, const Cfg& cfg,
                                T* g) {
 #pragma unroll
   for (int k = 0; k < C; ++k) {
-    g[k] = (r < H - 1) ? (T)(cfg.w[k] * cfg.ih0) * (vdown[k] - v[k]) : T(0);
-    g[C + k] = (c < W - 1) ? (T)(cfg.w[k] * cfg.ih1) * (vright[k] - v[k]) : T(0);
+    g[k] = (!M || r < H - 1) ? (T)(cfg.w[k] * cfg.ih0) * (vdown[k] - v[k]) : T(0);
+    g[C + k] = (!M || c < W - 1) ? (T)(cfg.w[k] * cfg.ih1) * (vright[k] - v[k]) : T(0);
   }
 }
 
 // y+ = proj_{|.| <= tvw}(y + sigma * grad(xbar)) at one pixel; g receives grad(xbar)
-template <typename T, int C>
-__device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W, T sigma,
-                                T tvw, const Cfg& cfg, T* yp, T* g) {
-  grad_px<T, C>(xb, xbd, xbr, r, c, H, W, cfg, g);
+template <typename T, int C, bool PC, bool M = true>
+__device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
+                                const DualStep<T, C, PC>& ds, T tvw, const Cfg& cfg, T* yp, T* g) {
+  grad_px<T, C, M>(xb, xbd, xbr, r, c, H, W, cfg, g);
   T n2 = T(0);
 #pragma unroll
   for (int e = 0; e < 2 * C; ++e) {
-    T s = sigma;
-    if (cfg.steps == OPY_TV_STEPS_PRECOND)
-      s = (e < C) ? precond_sigma<T>(r < H - 1, e, cfg.ih0, cfg) : precond_sigma<T>(c < W - 1, e - C, cfg.ih1, cfg);
-    yp[e] = y[e] + s * g[e];
+    yp[e] = y[e] + ds.template at<M>(e, r, c, H, W) * g[e];
     n2 += yp[e] * yp[e];
   }
-  if (n2 > tvw * tvw) {
-    const T sc = tvw / sqrt(n2);
+  if constexpr (sizeof(T) == 4) {
+    // branch-free: the scale is 1 (exact) inside the ball
+    const T sc = n2 > tvw * tvw ? ball_scale(tvw, n2) : T(1);
+#pragma unroll
+    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+  } else if (n2 > tvw * tvw) {
+    const T sc = ball_scale(tvw, n2);
 #pragma unroll
     for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
   }

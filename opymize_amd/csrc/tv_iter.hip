@@ -17,7 +17,7 @@ namespace {
 // array brings those columns in, and the edge lanes pick their row j up with
 // readlane plus a DPP shift.  Every access of the main data is aligned and
 // no lane runs code of its own.
-template <typename T, int C, int RPT, bool kAdapt>
+template <typename T, int C, int RPT, bool kAdapt, bool PC>
 __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin, T* __restrict__ xout,
                                                       const T* __restrict__ yin, T* __restrict__ yout,
                                                       const T* __restrict__ f, const opy_tv_inst_t* __restrict__ insts,
@@ -47,6 +47,8 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
     const double th_d = accel_theta(cfg.steps, I, tau_s);
     const T theta = (T)th_d;                      // 1 unless accelerated
     const T sigma = (T)(st[tr.b].sigma / th_d);
+    const PrimalStep<T, C, PC> prox(tau_s, I, cfg);
+    const DualStep<T, C, PC> ds(sigma, cfg);
     const T tvw = (T)I.tvw;
     const int c0 = tr.c0;
     const int c = c0 + lane;
@@ -118,10 +120,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
       T aty[C];
       aty_row(j, aty);
 #pragma unroll
-      for (int k = 0; k < C; ++k) {
-        const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(r, c, H, W, k, cfg) : (T)tau_s;
-        xp[j][k] = prox_data<T>(xv[j][k] - tau * aty[k], fv[j][k], (double)tau * I.lam, I.data);
-      }
+      for (int k = 0; k < C; ++k) xp[j][k] = prox(xv[j][k], aty[k], fv[j][k], r, c, H, W, k, cfg);
     }
     // ---- right halo, transposed: lane k forms x+ at (x row k, cr)
     T xbh[C], xdh[C];
@@ -142,8 +141,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
       adjoint_px<T, C>(yo, yo + C, yrh, yl1, rx, cr, H, W, cfg, ah);
 #pragma unroll
       for (int k = 0; k < C; ++k) {
-        const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(rx, cr, H, W, k, cfg) : (T)tau_s;
-        const T p = prox_data<T>(xrh[k] - tau * ah[k], frh[k], (double)tau * I.lam, I.data);
+        const T p = prox(xrh[k], ah[k], frh[k], rx, cr, H, W, k, cfg);
         xbh[k] = (T(1) + theta) * p - theta * xrh[k];
         xdh[k] = xrh[k] - p;
       }
@@ -164,7 +162,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
       adjoint_px<T, C>(yo, yo + C, ylh, yl1, rx, cl, H, W, cfg, ah);
 #pragma unroll
       for (int k = 0; k < C; ++k) {
-        const T p = prox_data<T>(xlh[k] - (T)tau_s * ah[k], flh[k], tau_s * I.lam, I.data);
+        const T p = prox(xlh[k], ah[k], flh[k], rx, cl, H, W, k, cfg);
         xbl[k] = (T(1) + theta) * p - theta * xlh[k];
       }
 #pragma unroll
@@ -178,7 +176,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         }
         xbc[k] = sel;
       }
-      yplus_px<T, C>(xbl, xbld, xbc, yo, rx, cl, H, W, sigma, tvw, cfg, ypl, gl);
+      yplus_px<T, C>(xbl, xbld, xbc, yo, rx, cl, H, W, ds, tvw, cfg, ypl, gl);
 #pragma unroll
       for (int k = 0; k < C; ++k) yplh[k] = ypl[C + k];
     }
@@ -200,7 +198,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         xbd[k] = xbar(i + 1, k);
         xbr[k] = dpp_shl1(readlane(xbh[k], i), xbi[k]);
       }
-      yplus_px<T, C>(xbi, xbd, xbr, yv[i + 1], r, c, H, W, sigma, tvw, cfg, yp, gb);
+      yplus_px<T, C>(xbi, xbd, xbr, yv[i + 1], r, c, H, W, ds, tvw, cfg, yp, gb);
       if (mine && ok) {
         const int64_t p = (int64_t)r * W + c;
         store_c<T, C>(xob + p * C, xp[i]);
@@ -274,19 +272,20 @@ int iter_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_i
     return fail(kInvalid, "tv_iter: adaptive steps exclude precond and accelerated steps");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
+#define OPY_L(CC, RR)                                                                                       \
+  hipLaunchKernelGGL((tv_iter_kernel<T, CC, RR, AD, PC>), dim3(launch_grid(KF(tv_iter_kernel<T, CC, RR, AD, PC>), grid)), \
+                     dim3(kTB), 0, stm, xi, xo, yi, yo, f, insts, st, ai, ao, na, c, partials)
   if (adapt) {
-#define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_iter_kernel<T, CC, RR, true>), dim3(launch_grid(KF(tv_iter_kernel<T, CC, RR, true>), grid)), \
-                     dim3(kTB), 0, stm, xi, xo, yi, yo, f, insts, st, ai, ao, na, c, partials)
+    constexpr bool AD = true, PC = false;
     OPY_TV_DISPATCH(OPY_L)
-#undef OPY_L
+  } else if (cfg->steps == OPY_TV_STEPS_PRECOND) {
+    constexpr bool AD = false, PC = true;
+    OPY_TV_DISPATCH(OPY_L)
   } else {
-#define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_iter_kernel<T, CC, RR, false>), dim3(launch_grid(KF(tv_iter_kernel<T, CC, RR, false>), grid)), \
-                     dim3(kTB), 0, stm, xi, xo, yi, yo, f, insts, st, ai, ao, na, c, partials)
+    constexpr bool AD = false, PC = false;
     OPY_TV_DISPATCH(OPY_L)
-#undef OPY_L
   }
+#undef OPY_L
   return check_launch("tv_iter");
 }
 

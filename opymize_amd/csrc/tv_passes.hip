@@ -6,7 +6,7 @@ namespace opy {
 namespace {
 
 // ---- pass 1 ---------------------------------------------------------------
-template <typename T, int C, int RPT>
+template <typename T, int C, int RPT, bool PC>
 __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xin, T* __restrict__ xout,
                                                         const T* __restrict__ y, const T* __restrict__ f,
                                                         const opy_tv_inst_t* __restrict__ insts,
@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xi
     const TileRef tr = locate(g, active_inst, act_off, nact, insts, tw, th);
     const opy_tv_inst_t I = insts[tr.b];
     const int H = I.H, W = I.W;
-    const double tau_s = st[tr.b].tau;
+    const PrimalStep<T, C, PC> prox(st[tr.b].tau, I, cfg);
     const int c = tr.c0 + tx;
     const bool col = c < W;
     const int rs = tr.r0 + ty * RPT;
@@ -76,10 +76,7 @@ __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xi
         T d[C], o[C];
         adjoint_px<T, C>(yv[i], yv[i] + C, i == 0 ? yu : yv[i - 1], yl1, r, c, H, W, cfg, d);
 #pragma unroll
-        for (int k = 0; k < C; ++k) {
-          const T tau = cfg.steps == OPY_TV_STEPS_PRECOND ? precond_tau<T>(r, c, H, W, k, cfg) : (T)tau_s;
-          o[k] = prox_data<T>(xv[i][k] - tau * d[k], fv[i][k], (double)tau * I.lam, I.data);
-        }
+        for (int k = 0; k < C; ++k) o[k] = prox(xv[i][k], d[k], fv[i][k], r, c, H, W, k, cfg);
         store_c<T, C>(ob + pix * C, o);
       }
     }
@@ -87,7 +84,7 @@ __global__ void __launch_bounds__(kTB) tv_primal_kernel(const T* __restrict__ xi
 }
 
 // ---- pass 2 ---------------------------------------------------------------
-template <typename T, int C, int RPT, bool kAdapt>
+template <typename T, int C, int RPT, bool kAdapt, bool PC>
 __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, const T* __restrict__ xn,
                                                       const T* __restrict__ yin, T* __restrict__ yout,
                                                       const opy_tv_inst_t* __restrict__ insts,
@@ -114,6 +111,7 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
     const double th_d = accel_theta(cfg.steps, I, st[tr.b].tau);
     const T theta = (T)th_d;                      // 1 unless accelerated
     const T sigma = (T)(st[tr.b].sigma / th_d);
+    const DualStep<T, C, PC> ds(sigma, cfg);
     const T tau = (T)st[tr.b].tau;
     const T tvw = (T)I.tvw;
     const int c = tr.c0 + tx;
@@ -210,7 +208,7 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
     for (int k = 0; k < C; ++k) ypu[k] = T(0);
     if (kAdapt) {
       T yp[2 * C], g[2 * C];
-      yplus_px<T, C>(bx[0], bx[1], bxr[0], yk[0], rs - 1, c, H, W, sigma, tvw, cfg, yp, g);
+      yplus_px<T, C>(bx[0], bx[1], bxr[0], yk[0], rs - 1, c, H, W, ds, tvw, cfg, yp, g);
 #pragma unroll
       for (int k = 0; k < C; ++k) ypu[k] = yp[k];
     }
@@ -220,7 +218,7 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
       const bool ok = col && r < H;
       const int64_t pix = (int64_t)r * W + c;
       T yp[2 * C], g[2 * C];
-      yplus_px<T, C>(bx[i + 1], bx[i + 2], bxr[i + 1], yk[i + 1], r, c, H, W, sigma, tvw, cfg, yp, g);
+      yplus_px<T, C>(bx[i + 1], bx[i + 2], bxr[i + 1], yk[i + 1], r, c, H, W, ds, tvw, cfg, yp, g);
       if (ok) store_c<T, 2 * C>(yob + pix * 2 * C, yp);
       if (kAdapt) {
         // t=1 parts of y_k and y+ of the left neighbour
@@ -232,7 +230,7 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
         }
         if (lhalo && ok && c > 0) {
           T ypl[2 * C], gl[2 * C];
-          yplus_px<T, C>(bxl[i], bxl[i + 1], bx[i + 1], ykl[i], r, c - 1, H, W, sigma, tvw, cfg, ypl, gl);
+          yplus_px<T, C>(bxl[i], bxl[i + 1], bx[i + 1], ykl[i], r, c - 1, H, W, ds, tvw, cfg, ypl, gl);
 #pragma unroll
           for (int k = 0; k < C; ++k) {
             ykl1[k] = ykl[i][C + k];
@@ -290,9 +288,15 @@ int primal_impl(const T* xin, T* xout, const T* y, const T* f, const opy_tv_inst
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
 #define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_primal_kernel<T, CC, RR>), dim3(launch_grid(KF(tv_primal_kernel<T, CC, RR>), grid)), dim3(kTB), 0, \
-                     stm, xin, xout, y, f, insts, st, ai, ao, na, c)
-  OPY_TV_DISPATCH(OPY_L)
+  hipLaunchKernelGGL((tv_primal_kernel<T, CC, RR, PC>), dim3(launch_grid(KF(tv_primal_kernel<T, CC, RR, PC>), grid)), \
+                     dim3(kTB), 0, stm, xin, xout, y, f, insts, st, ai, ao, na, c)
+  if (cfg->steps == OPY_TV_STEPS_PRECOND) {
+    constexpr bool PC = true;
+    OPY_TV_DISPATCH(OPY_L)
+  } else {
+    constexpr bool PC = false;
+    OPY_TV_DISPATCH(OPY_L)
+  }
 #undef OPY_L
   return check_launch("tv_primal");
 }
@@ -310,19 +314,20 @@ int dual_impl(const T* xo, const T* xn, const T* yi, T* yo, const opy_tv_inst_t*
     return fail(kInvalid, "tv_dual: adaptive steps exclude precond and accelerated steps");
   Cfg c = make_cfg(cfg);
   hipStream_t stm = as_stream(s);
+#define OPY_L(CC, RR)                                                                                       \
+  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, AD, PC>), dim3(launch_grid(KF(tv_dual_kernel<T, CC, RR, AD, PC>), grid)), \
+                     dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
   if (adapt) {
-#define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, true>), dim3(launch_grid(KF(tv_dual_kernel<T, CC, RR, true>), grid)), \
-                     dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
+    constexpr bool AD = true, PC = false;
     OPY_TV_DISPATCH(OPY_L)
-#undef OPY_L
+  } else if (cfg->steps == OPY_TV_STEPS_PRECOND) {
+    constexpr bool AD = false, PC = true;
+    OPY_TV_DISPATCH(OPY_L)
   } else {
-#define OPY_L(CC, RR) \
-  hipLaunchKernelGGL((tv_dual_kernel<T, CC, RR, false>), dim3(launch_grid(KF(tv_dual_kernel<T, CC, RR, false>), grid)), \
-                     dim3(kTB), 0, stm, xo, xn, yi, yo, insts, st, ai, ao, na, c, partials)
+    constexpr bool AD = false, PC = false;
     OPY_TV_DISPATCH(OPY_L)
-#undef OPY_L
   }
+#undef OPY_L
   return check_launch("tv_dual");
 }
 

@@ -7,7 +7,10 @@ and, with adaptive steps, one small per-instance rule kernel; every
 `block_iters` iterations it evaluates the FP64 primal-dual gap, marks
 converged instances and compacts
 the list of running instances -- all on the device, so a whole block is one
-hipGraph replay and the host only reads the number of running instances.
+hipGraph replay and the host only reads the number of running instances and
+their tiles, which sizes the next launches (one block per running tile: the
+hardware dispatcher then keeps the tiles in flight contiguous in memory,
+which streams faster than a persistent grid whose blocks drift apart).
 
 Model of instance b: min_x G_b(x) + tvw_b * sum_p |grad x(p)|_F with
 G_b = 0.5*lam_b*|x - f_b|^2 (data="l2") or lam_b*|x - f_b|_1 (data="l1"),
@@ -54,6 +57,7 @@ _SIGS = {
     "opy_tv_primal": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_dual": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_iter": [ctypes.c_void_p] * 11 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_multi": [ctypes.c_void_p] * 11 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_gap": [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_reset": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4,
 }
@@ -101,10 +105,16 @@ class TVBatch(object):
     passes: 1 = one kernel per iteration (reads x, f, y; writes x+, y+),
             2 = a primal and a dual pass (reads x, f, y, then x, x+, y)
     tile:   (tile width, rows per thread); with passes=1 the width is 64
+    multi:  iterations per pass over HBM with the constant, preconditioned
+            and accelerated step rules (passes=1): K in {2, 3, 4} runs K
+            iterations per launch on LDS-staged tiles (rows per thread 2, 4
+            or 8), 1 one iteration per launch; adaptive steps always take
+            one iteration per launch (their rule needs every iteration's
+            residuals)
     """
 
     def __init__(self, images, lam, data="l2", tvw=1.0, precision="float", tile=(64, 4), h=(1.0, 1.0),
-                 weights=None, grid=None, passes=1):
+                 weights=None, grid=None, passes=1, multi=1):
         _hip.require_device()
         _bind()
         self.B = len(images)
@@ -143,10 +153,19 @@ class TVBatch(object):
         if passes not in (1, 2):
             raise ValueError("passes must be 1 or 2")
         self.passes = passes
+        self.set_multi(multi)
         self.set_tile(*tile)
         self.graph = None
+        self.cur = 0  # buffer holding the current iterate
 
     # -- layout --------------------------------------------------------------
+    def set_multi(self, k):
+        if int(k) not in (1, 2, 3, 4):
+            raise ValueError("multi must be 1, 2, 3 or 4 iterations per launch")
+        if int(k) > 1 and self.passes != 1:
+            raise ValueError("multi > 1 needs passes=1")
+        self.multi = int(k)
+
     def set_tile(self, tw, rpt):
         if self.passes == 1:
             tw = 64
@@ -173,9 +192,30 @@ class TVBatch(object):
         self.act_off = torch.zeros(self.B + 1, dtype=torch.int32, device="cuda")
         self.nact = torch.zeros(1, dtype=torch.int32, device="cuda")
         self.iter_counter = torch.zeros(1, dtype=torch.int64, device="cuda")
-        # 0: the library sizes each persistent launch to the blocks resident at once
-        self.grid = int(max(1, min(self._grid_override, t0))) if self._grid_override else 0
+        # grid=0 makes the library size a persistent launch to the blocks
+        # resident at once; by default (no override) every launch gets one
+        # block per tile of the running instances, rounded up to a bucket
+        # (see _launch_grid), which keeps the tiles in flight contiguous
+        self.grid = int(max(1, min(self._grid_override, t0))) if self._grid_override is not None else t0
         self.graph = None
+
+    def _launch_grid(self, tiles):
+        """Blocks per launch for `tiles` running tiles: one per tile, rounded
+        up to a multiple of a sixteenth of the batch (at least 1024) so that a
+        solve needs few distinct launch shapes (one captured graph each).
+        Blocks past the running tiles return at once."""
+        if self._grid_override is not None:
+            return self.grid
+        q = max(1024, -(-self.total_tiles // 16))
+        return int(max(1, min(self.total_tiles, -(-max(tiles, 1) // q) * q)))
+
+    def refresh_grid(self):
+        """Read the running tile count (one device read) and size the
+        following launches to it; returns the number of running instances."""
+        nrun = int(self.nact.item())
+        tiles = int(self.act_off[nrun].item()) if nrun > 0 else 0
+        self.grid = self._launch_grid(tiles)
+        return nrun
 
     def _cfg(self, steps):
         c = TVCfg()
@@ -189,6 +229,7 @@ class TVBatch(object):
     # -- state -----------------------------------------------------------------
     def set_start(self, x0=None, y0=None):
         """Packed start values (device tensors or host arrays of the packed size), zeros otherwise."""
+        self.cur = 0
         for t, v in ((self.x[0], x0), (self.y[0], y0)):
             if v is None:
                 t.zero_()
@@ -268,7 +309,7 @@ class TVBatch(object):
     def _gap_check(self, cfg, tol, tol_infeas, certify, iters_in_block):
         P = _hip.ptr
         s = _hip.stream()
-        self._call("opy_tv_gap" + self._sfx(), P(self.x[0]), P(self.y[0]), P(self.f), P(self.insts),
+        self._call("opy_tv_gap" + self._sfx(), P(self.x[self.cur]), P(self.y[self.cur]), P(self.f), P(self.insts),
                    P(self.active_inst), P(self.act_off), P(self.nact), ctypes.byref(cfg), P(self.part_gap),
                    self.grid, s)
         self._call("opy_tv_check", P(self.insts), P(self.checks), P(self.part_gap), self.B, float(tol),
@@ -279,8 +320,24 @@ class TVBatch(object):
         self._call("opy_tv_compact", P(self.insts), P(self.checks), self.B, P(self.active_inst), P(self.act_off),
                    P(self.nact), P(self.iter_counter), int(iters_in_block), _hip.stream())
 
+    def _multi_launch(self, p, cfg, k):
+        """k iterations in one launch from buffer p, then the k step updates."""
+        P = _hip.ptr
+        s = _hip.stream()
+        self._call("opy_tv_multi" + self._sfx(), P(self.x[p]), P(self.x[1 - p]), P(self.y[p]), P(self.y[1 - p]),
+                   P(self.f), P(self.insts), P(self.state), P(self.active_inst), P(self.act_off), P(self.nact),
+                   ctypes.byref(cfg), int(k), self.grid, s)
+        if cfg.steps == STEP_MODES["accel"]:
+            for _ in range(k):
+                self._call("opy_tv_accel", P(self.insts), P(self.state), P(self.checks), self.B, s)
+
     def _iterations(self, n, cfg, adapt, params):
         """n iterations starting from buffer self.cur (updates self.cur)."""
+        k = self.multi if not adapt else 1
+        while k > 1 and n >= k:
+            self._multi_launch(self.cur, cfg, k)
+            self.cur ^= 1
+            n -= k
         for _ in range(n):
             self._iteration(self.cur, cfg, adapt, params)
             self.cur ^= 1
@@ -288,6 +345,36 @@ class TVBatch(object):
     def _check_step(self, cfg, tol, tol_infeas, certify, iters_since):
         self._gap_check(cfg, tol, tol_infeas, certify, iters_since)
         self._compact(iters_since)
+
+    def _graph_block(self, cache, fn):
+        """Run fn (a block of launches) through a graph captured once per
+        (launch grid, buffer holding the iterate at the start).  The first
+        block of a solve runs eagerly, which loads its kernels before any
+        capture; a new key is captured (that records without running) and
+        then replayed."""
+        key = (self.grid, self.cur)
+        if key in cache:
+            g, end = cache[key]
+            g.replay()
+            self.cur = end
+        elif not cache["warm"]:
+            fn()
+            torch.cuda.synchronize()
+            cache["warm"] = True
+        else:
+            start = self.cur
+            g = self._capture(fn)
+            end, self.cur = self.cur, start
+            cache[key] = (g, end)
+            g.replay()
+            self.cur = end
+
+    def _to_buffer0(self):
+        """Bring the current iterate to buffer 0 (the start and result buffer)."""
+        if self.cur != 0:
+            for a in (self.x, self.y):
+                _hip.call("copy", self.dtype, _hip.ptr(a[1]), _hip.ptr(a[0]), a[0].numel(), _hip.stream())
+            self.cur = 0
 
     @staticmethod
     def _capture(fn):
@@ -312,7 +399,7 @@ class TVBatch(object):
         """Reset the flagged instances to x = y = 0 with fresh step state."""
         fl = torch.as_tensor(np.asarray(flags, dtype=np.int32)).cuda()
         self._call("opy_tv_reset" + self._sfx(), _hip.ptr(self.insts), _hip.ptr(fl), self.B, self.C,
-                   _hip.ptr(self.x[0]), _hip.ptr(self.y[0]), _hip.ptr(self.checks), _hip.stream())
+                   _hip.ptr(self.x[self.cur]), _hip.ptr(self.y[self.cur]), _hip.ptr(self.checks), _hip.stream())
         st = self.state.cpu().numpy()
         F = _hip.ADAPT_FIELD
         for b in np.flatnonzero(flags):
@@ -361,42 +448,29 @@ class TVBatch(object):
                 if bad.any():
                     self.fallbacks = list(np.flatnonzero(bad))
                     self._cold_restart(bad, steps, step_bound, step_factor)
-            if int(self.nact.item()) == 0:
+            if self.refresh_grid() == 0:
                 self.iters_run = 0
                 return self.results()
-        # graphs need an even block (the buffer roles return to their start)
-        graphs = use_graph and block_iters % 2 == 0
-        g_it = g_ck = None
+        else:
+            self.refresh_grid()
+        graphs = use_graph
+        # captured blocks per (launch grid, buffer holding the iterate at the start)
+        g_it, g_ck = {"warm": False}, {"warm": False}
         done_iters, nblk = 0, 0
         since = 0
         while done_iters < max_iters:
             n = min(block_iters, max_iters - done_iters)
             if graphs and n == block_iters:
-                if g_it is None:
-                    self._iterations(n, cfg, adapt, params)  # eager warm-up block, counted
-                    torch.cuda.synchronize()
-                    g_it = self._capture(lambda: self._iterations(n, cfg, adapt, params))
-                else:
-                    g_it.replay()
+                self._graph_block(g_it, lambda: self._iterations(n, cfg, adapt, params))
             else:
                 self._iterations(n, cfg, adapt, params)
             done_iters += n
             since += n
             nblk += 1
             if nblk % check_every == 0 or done_iters >= max_iters:
-                if self.cur != 0:
-                    # odd number of iterations: bring the current iterate to buffer 0
-                    for a in (self.x, self.y):
-                        _hip.call("copy", self.dtype, _hip.ptr(a[1]), _hip.ptr(a[0]), a[0].numel(), _hip.stream())
-                    self.cur = 0
                 if graphs and since == block_iters * check_every:
-                    if g_ck is None:
-                        self._check_step(cfg, tol, tol_infeas, certify, since)
-                        torch.cuda.synchronize()
-                        g_ck = self._capture(lambda: self._check_step(cfg, tol, tol_infeas, certify,
-                                                                      block_iters * check_every))
-                    else:
-                        g_ck.replay()
+                    self._graph_block(g_ck, lambda: self._check_step(cfg, tol, tol_infeas, certify,
+                                                                     block_iters * check_every))
                 else:
                     self._check_step(cfg, tol, tol_infeas, certify, since)
                 since = 0
@@ -410,11 +484,12 @@ class TVBatch(object):
                         self.fallbacks += list(np.flatnonzero(flags))
                         self._cold_restart(flags, steps, step_bound, step_factor)
                     gap0 = None
-                nrun = int(self.nact.item())
+                nrun = self.refresh_grid()
                 stop = callback(self, done_iters) if callback is not None else False
                 if nrun == 0 or stop:
                     break
         self.iters_run = done_iters
+        self._to_buffer0()
         return self.results()
 
     def results(self):
@@ -436,4 +511,4 @@ class TVBatch(object):
 
     def solution(self):
         """Per-instance (x, y) host arrays of the current iterates (buffer 0)."""
-        return self.unpack(self.x[0], "x"), self.unpack(self.y[0], "y")
+        return self.unpack(self.x[self.cur], "x"), self.unpack(self.y[self.cur], "y")

@@ -1,0 +1,177 @@
+"""Several fused TV iterations per launch (opy_tv_multi, TVBatch(multi=K)):
+the iterates of K iterations on LDS-staged tiles against the oracle and
+against K single-iteration launches.  Tolerances as in test_gpu_tv_batch.py
+(FP64 1e-9, FP32 5e-4 max abs against the oracle); against the single-launch
+path 1e-6 (FP32) and 1e-12 (FP64): the per-pixel arithmetic is the same, so
+only the compiler's instruction selection can differ."""
+import numpy as np
+import pytest
+
+from oracle import pdhg as opd
+
+from test_gpu_tv_batch import SHAPES, make_batch, rects
+
+pytestmark = pytest.mark.gpu
+
+
+def lds_bytes(precision, C, rpt, k):
+    """LDS of one block: y and xbar (3C values) over the tile and k pixels
+    around it, rows rounded up to a multiple of 4, and a guard pixel on every
+    side."""
+    rows = 4 * ((4 * rpt + 2 * k + 3) // 4) + 2
+    return rows * (64 + 2 * k + 2) * 3 * C * (8 if precision == "double" else 4)
+
+
+def _batch(precision, tile, multi, shapes=SHAPES, C=1, seed=0, data=None):
+    from opymize_amd.solvers.tv import TVBatch
+    rng = np.random.default_rng(seed)
+    if C == 1:
+        imgs = [rects(H, W, seed + i) for i, (H, W) in enumerate(shapes)]
+    else:
+        imgs = [np.stack([rects(H, W, 10 * C + i + k) for k in range(C)], axis=2) for i, (H, W) in enumerate(shapes)]
+    lams = list(np.round(np.exp(rng.uniform(0.0, np.log(20.0), len(shapes))), 3))
+    if data is None:
+        data = ["l2" if i % 3 != 1 else "l1" for i in range(len(shapes))]
+    return TVBatch(imgs, lams, data, precision=precision, tile=tile, multi=multi), imgs, lams, data
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("steps", ["const", "precond", "accel"])
+@pytest.mark.parametrize("multi,rpt", [(2, 8), (3, 4), (4, 2), (4, 8)])
+def test_multi_matches_oracle(gpu, precision, steps, multi, rpt):
+    """40 iterations in blocks of 20 (K=3: six launches of 3 and two single
+    iterations per block) on the uneven batch, including 1-row, 5-row and
+    5-column images."""
+    from opymize_amd import _hip
+    bat, imgs, lams, data = _batch(precision, (64, rpt), multi)
+    k = 40
+    if lds_bytes(precision, 1, rpt, multi) > 64 * 1024:  # FP64 with 8 rows per thread and K = 3 or 4
+        with pytest.raises(_hip.HipError, match="LDS"):
+            bat.solve(steps=steps, step_bound=0.124, max_iters=k, block_iters=20, tol=1e-30)
+        return
+    bat.solve(steps=steps, step_bound=0.124, max_iters=k, block_iters=20, tol=1e-30)
+    xs, ys = bat.solution()
+    res = bat.results()
+    st = bat.steps_state()
+    tol = 1e-9 if precision == "double" else 5e-4
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps=steps, step_bound=0.124, term_maxiter=k, granularity=k,
+                                  term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < tol, b
+        assert np.abs(ys[b] - yo).max() < tol, b
+        assert res[b]["iter"] == k and res[b]["status"] == 0
+        if steps == "accel":
+            assert abs(st[b]["tau"] - io["tau"]) <= 1e-12 * io["tau"], b
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("multi,rpt", [(2, 8), (3, 4), (4, 4)])
+@pytest.mark.parametrize("steps", ["accel", "precond"])
+def test_multi_equals_single_launches(gpu, precision, multi, rpt, steps):
+    """K iterations per launch give the iterates of K single-iteration
+    launches (the same tile map and per-pixel arithmetic)."""
+    shapes = [(130, 200), (64, 64), (33, 257), (1, 100), (97, 1)]
+    out = {}
+    for m in (1, multi):
+        bat, *_ = _batch(precision, (64, rpt), m, shapes=shapes, seed=7)
+        bat.solve(steps=steps, step_bound=0.124, max_iters=24, block_iters=12, tol=1e-30)
+        out[m] = bat.solution()
+    tol = 1e-12 if precision == "double" else 1e-6
+    for a, b in zip(out[1][0] + out[1][1], out[multi][0] + out[multi][1]):
+        assert np.abs(a - b).max() <= tol
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("C", [2, 3, 4])
+def test_multi_channels(gpu, precision, C):
+    """Per-pixel vectors of C and 2C values in the LDS region; in FP64 the
+    smallest region (rows per thread 2, K = 2) exceeds the LDS budget from
+    C = 3 on and is refused."""
+    from opymize_amd import _hip
+    shapes = [(33, 150), (9, 300), (70, 41)]
+    if lds_bytes(precision, C, 2, 2) > 64 * 1024:
+        with pytest.raises(_hip.HipError, match="LDS"):
+            bat, *_ = _batch(precision, (64, 2), 2, shapes=shapes, C=C)
+            bat.solve(steps="precond", step_bound=0.124, max_iters=4, block_iters=4, tol=1e-30)
+        return
+    bat, imgs, lams, data = _batch(precision, (64, 2), 2, shapes=shapes, C=C)
+    k = 30
+    bat.solve(steps="precond", step_bound=0.124, max_iters=k, block_iters=10, tol=1e-30)
+    xs, ys = bat.solution()
+    tol = 1e-9 if precision == "double" else 5e-4
+    for b, im in enumerate(imgs):
+        H, W = im.shape[:2]
+        prob = opd.tv_problem(im.reshape(-1, C), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps="precond", step_bound=0.124, term_maxiter=k, granularity=k,
+                                  term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < tol, b
+        assert np.abs(ys[b] - yo).max() < tol, b
+
+
+def test_multi_graph_blocks_with_odd_launch_counts(gpu):
+    """Blocks of 10 iterations with K = 2 are 5 launches, so consecutive
+    blocks start from alternating buffers; graph replays per start buffer
+    give the eager result."""
+    out = {}
+    for g in (False, True):
+        bat, *_ = make_batch("double")
+        bat.set_multi(2)
+        bat.set_tile(64, 4)
+        bat.solve(steps="accel", step_bound=0.124, max_iters=70, block_iters=10, tol=1e-30, use_graph=g)
+        out[g] = bat.solution()
+        assert bat.cur == 0
+    for a, b in zip(out[False][0] + out[False][1], out[True][0] + out[True][1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_multi_converges_and_certifies(gpu):
+    """A whole solve with K = 4 certifies every instance, and the strong
+    convexity bound holds against the oracle's converged solution."""
+    bat, imgs, lams, data = _batch("float", (64, 8), 4, seed=3, data="l2")
+    res = bat.solve(steps="accel", step_bound=0.124, max_iters=20000, block_iters=40, tol=1e-5)
+    xs, _ = bat.solution()
+    assert all(r["status"] == 1 for r in res)
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], "l2")
+        xo, _, io, _ = opd.solve(prob, steps="accel", step_bound=0.124, term_maxiter=20000, granularity=100,
+                                 term_pd_gap=1e-9)
+        # 0.5 lam |x - x*|^2 <= gap for either point
+        bound = np.sqrt(2 * abs(res[b]["gap"]) / lams[b]) + np.sqrt(2 * abs(io["gap"]) / lams[b])
+        assert np.linalg.norm(xs[b] - xo) <= bound + 1e-4, b
+
+
+def test_multi_rejects_bad_arguments(gpu):
+    from opymize_amd import _hip
+    from opymize_amd.solvers.tv import TVBatch
+    bat, *_ = make_batch("float", (64, 6))
+    with pytest.raises(ValueError):
+        bat.set_multi(5)
+    with pytest.raises(ValueError):
+        TVBatch([np.zeros((8, 8))], 1.0, passes=2, multi=2)
+    bat.set_multi(2)  # rows per thread 6 is not a multi-iteration tile
+    with pytest.raises(_hip.HipError, match="rows per thread"):
+        bat.solve(steps="const", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30)
+
+
+def test_autotune_considers_iterations_per_launch(gpu):
+    """With accelerated steps the tuner also times several iterations per
+    launch; the winner leaves iterates that match the oracle."""
+    from opymize_amd.solvers import autotune
+    bat, imgs, lams, data = make_batch("double", (64, 4))
+    autotune._cache.clear()
+    best = autotune.tune(bat, steps="accel", iters=12, blocks=(12, 24))
+    assert best["multi"] in autotune.MULTI and bat.multi == best["multi"]
+    assert (bat.tw, bat.rpt) == tuple(best["tile"]) and best["block_iters"] in (12, 24)
+    k = 24
+    bat.solve(steps="accel", step_bound=0.124, max_iters=k, block_iters=12, tol=1e-30)
+    xs, _ = bat.solution()
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, _, _, _ = opd.solve(prob, steps="accel", step_bound=0.124, term_maxiter=k, granularity=k,
+                                term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < 1e-9, b
+    autotune._cache.clear()
