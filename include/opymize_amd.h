@@ -324,7 +324,7 @@ int opy_tv_iter_f64(const double* x_in, double* x_out, const double* y_in, doubl
                     const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
                     const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int adapt,
                     double* partials, int grid, void* stream);
-/* k in {2,3,4} whole iterations in one pass over HBM (cfg->tw = 64, cfg->rpt in {2,4,8}; no
+/* k in {2,3,4} whole iterations in one pass over HBM (cfg->tw = 64, cfg->rpt in {2,4,6,8}; no
  * adaptive steps, whose rule needs every iteration's residuals): the same iterates as k
  * opy_tv_iter launches with steps SCALAR / PRECOND / ACCEL (PDHG.iteration_step, pdhg.py:124-153,
  * k times).  Each block loads its tile and k pixels around it into LDS and iterates there.  The

@@ -227,8 +227,8 @@ int multi_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_
                const opy_tv_cfg_t* cfg, int k, int grid, void* s) {
   if (int rc = check_cfg(cfg)) return rc;
   if (cfg->tw != 64) return fail(kInvalid, "tv_multi: needs tw == 64 (got %d)", cfg->tw);
-  if (cfg->rpt != 2 && cfg->rpt != 4 && cfg->rpt != 8)
-    return fail(kInvalid, "tv_multi: rows per thread %d not in {2,4,8}", cfg->rpt);
+  if (cfg->rpt != 2 && cfg->rpt != 4 && cfg->rpt != 6 && cfg->rpt != 8)
+    return fail(kInvalid, "tv_multi: rows per thread %d not in {2,4,6,8}", cfg->rpt);
   if (k != 2 && k != 3 && k != 4) return fail(kInvalid, "tv_multi: k = %d not in {2,3,4}", k);
   if (multi_lds<T>(cfg->C, cfg->rpt, k) > kMultiLdsMax)
     return fail(kInvalid, "tv_multi: C=%d, rpt=%d, k=%d needs %d B of LDS (at most %d)", cfg->C, cfg->rpt, k,
@@ -249,6 +249,7 @@ int multi_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_
   switch (cfg->rpt) {               \
     case 2: OPY_MK(CC, 2); break;   \
     case 4: OPY_MK(CC, 4); break;   \
+    case 6: OPY_MK(CC, 6); break;   \
     default: OPY_MK(CC, 8); break;  \
   }
 #define OPY_MC                     \

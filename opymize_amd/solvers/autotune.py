@@ -17,7 +17,7 @@ import torch
 TILE_WIDTHS = (32, 64, 128, 256)          # two-pass kernels
 ROWS_PER_THREAD = (1, 2, 4, 8)
 ITER_ROWS_PER_THREAD = (4, 6, 8, 12, 16)  # single-pass kernel (tile width 64)
-MULTI_ROWS_PER_THREAD = (2, 4, 8)         # several iterations per launch
+MULTI_ROWS_PER_THREAD = (2, 4, 6, 8)         # several iterations per launch
 MULTI = (1, 2, 3, 4)                      # iterations per launch (constant / precond / accel steps)
 BLOCK_ITERS = (12, 24, 48, 96)
 

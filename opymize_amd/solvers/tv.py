@@ -107,8 +107,8 @@ class TVBatch(object):
     tile:   (tile width, rows per thread); with passes=1 the width is 64
     multi:  iterations per pass over HBM with the constant, preconditioned
             and accelerated step rules (passes=1): K in {2, 3, 4} runs K
-            iterations per launch on LDS-staged tiles (rows per thread 2, 4
-            or 8), 1 one iteration per launch; adaptive steps always take
+            iterations per launch on LDS-staged tiles (rows per thread 2, 4,
+            6 or 8), 1 one iteration per launch; adaptive steps always take
             one iteration per launch (their rule needs every iteration's
             residuals)
     """
