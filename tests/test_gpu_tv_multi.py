@@ -146,12 +146,12 @@ def test_multi_converges_and_certifies(gpu):
 def test_multi_rejects_bad_arguments(gpu):
     from opymize_amd import _hip
     from opymize_amd.solvers.tv import TVBatch
-    bat, *_ = make_batch("float", (64, 6))
+    bat, *_ = make_batch("float", (64, 12))
     with pytest.raises(ValueError):
         bat.set_multi(5)
     with pytest.raises(ValueError):
         TVBatch([np.zeros((8, 8))], 1.0, passes=2, multi=2)
-    bat.set_multi(2)  # rows per thread 6 is not a multi-iteration tile
+    bat.set_multi(2)  # rows per thread 12 is a single-pass tile, not a multi-iteration one
     with pytest.raises(_hip.HipError, match="rows per thread"):
         bat.solve(steps="const", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30)
 
