@@ -145,6 +145,26 @@ int opy_matmul_r_f64(const double* x, double* y, const double* A, int64_t N, int
 /* DiagMatrixMultR: y[i,k] (+)= x[i,k] A[k]; einsum.py:104-128 */
 int opy_diag_matmul_r_f32(const float* x, float* y, int64_t N, int K, const float* A, int add, void* stream);
 int opy_diag_matmul_r_f64(const double* x, double* y, int64_t N, int K, const double* A, int add, void* stream);
+/* lifting operators.  IndexedMult (adj=0): y[j,i,m] (+)= -sum_l B[j,m,l] x[i,P[j,l]], x (N,K) -> y (J,N,M);
+ * IndexedMultAdj (adj=1): y[i,P[j,l]] (+)= -sum_m B[j,m,l] x[j,i,m], x (J,N,M) -> y (N,K).  B (J,M,L) in the
+ * element type, P (J,L) int32 in [0,K) (entries outside are skipped).
+ * IndexedMult / IndexedMultAdj, indexed.py:34-94, indexed.cu:1-48 */
+int opy_indexedmult_f32(const float* x, float* y, const float* B, const int32_t* P, int64_t N, int J, int M, int L,
+                        int K, int adj, int add, void* stream);
+int opy_indexedmult_f64(const double* x, double* y, const double* B, const int32_t* P, int64_t N, int J, int M, int L,
+                        int K, int adj, int add, void* stream);
+/* TangledMatrixMultR: y[m,i,k] (+)= sum_jl x[j,i,l] A(j,l,m,k), x (J,N,L) -> y (M,N,K); A is J x L x M x K, or
+ * M x K x J x L with trans=1 (the adjoint); einsum.py:130-173, einsum.cu:90-123 */
+int opy_tangled_matmul_r_f32(const float* x, float* y, const float* A, int64_t N, int J, int L, int M, int K, int trans,
+                             int add, void* stream);
+int opy_tangled_matmul_r_f64(const double* x, double* y, const double* A, int64_t N, int J, int L, int M, int K,
+                             int trans, int add, void* stream);
+/* MatrixMultRBatched: y[j,i,k] (+)= sum_m x[j,i,m] A(j,m,k), x (J,N,M) -> y (J,N,K); A is J x M x K, or J x K x M
+ * with trans=1 (the adjoint); einsum.py:175-214, einsum.cu:60-89 */
+int opy_matmul_r_batched_f32(const float* x, float* y, const float* A, int64_t N, int J, int M, int K, int trans,
+                             int add, void* stream);
+int opy_matmul_r_batched_f64(const double* x, double* y, const double* A, int64_t N, int J, int M, int K, int trans,
+                             int add, void* stream);
 /* CSR product for LinOps that only have a matrix (LinOp._call_cpu spmat.dot, __init__.py:95-108) */
 int opy_csr_spmv_f32(int64_t nrows, const int64_t* indptr, const int64_t* indices, const float* data, const float* x,
                      float* y, int add, void* stream);

@@ -30,6 +30,16 @@ class DeviceConstants(object):
             self._store[key] = t
         return t
 
+    def get_i32(self, name, host):
+        dev = torch.cuda.current_device()
+        key = (name, torch.int32, dev)
+        t = self._store.get(key)
+        if t is None:
+            _hip.require_device()
+            t = torch.as_tensor(np.ascontiguousarray(np.asarray(host, dtype=np.int32))).to("cuda")
+            self._store[key] = t
+        return t
+
     def drop(self, name=None):
         if name is None:
             self._store.clear()

@@ -88,6 +88,9 @@ _SIGS = {
     "matmul": [_P, _P, _P, _L, _I, _I, _I, _I, _P],
     "matmul_r": [_P, _P, _P, _L, _I, _I, _I, _I, _P],
     "diag_matmul_r": [_P, _P, _L, _I, _P, _I, _P],
+    "indexedmult": [_P, _P, _P, _P, _L, _I, _I, _I, _I, _I, _I, _P],
+    "tangled_matmul_r": [_P, _P, _P, _L, _I, _I, _I, _I, _I, _I, _P],
+    "matmul_r_batched": [_P, _P, _P, _L, _I, _I, _I, _I, _I, _P],
     "csr_spmv": [_L, _P, _P, _P, _P, _P, _I, _P],
     "csr_rowwise": [_L, _P, _P, _D, _P, _I, _P],
     "fill": [_P, _L, _D, _P],

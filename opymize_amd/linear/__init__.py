@@ -4,7 +4,9 @@ import torch
 
 from opymize_amd import LinOp, Variable, _hip
 from opymize_amd.linear.diff import GradientOp, DivergenceOp, LaplacianOp
-from opymize_amd.linear.einsum import MatrixMult, MatrixMultR, MatrixOp, DiagMatrixMultR
+from opymize_amd.linear.einsum import (MatrixMult, MatrixMultR, MatrixOp, DiagMatrixMultR, TangledMatrixMultR,
+                                       MatrixMultRBatched)
+from opymize_amd.linear.indexed import IndexedMult, IndexedMultAdj
 from opymize_amd.linear.scale import ZeroOp, ScaleOp, IdentityOp, NihilOp
 from opymize_amd.linear.sparse import CsrOp
 
@@ -125,4 +127,5 @@ class BlockOp(LinOp):
 
 
 __all__ = ["normest", "BlockOp", "GradientOp", "DivergenceOp", "LaplacianOp", "MatrixMult", "MatrixMultR",
-           "MatrixOp", "DiagMatrixMultR", "ZeroOp", "ScaleOp", "IdentityOp", "NihilOp", "CsrOp"]
+           "MatrixOp", "DiagMatrixMultR", "TangledMatrixMultR", "MatrixMultRBatched", "IndexedMult", "IndexedMultAdj",
+           "ZeroOp", "ScaleOp", "IdentityOp", "NihilOp", "CsrOp"]

@@ -149,3 +149,92 @@ def matmul_r_matrix(A, N, trans=False):
     """MatrixMultR as a sparse matrix acting on flat (N, K) -> (N, J)."""
     Am = A.T if trans else A  # (K, J)
     return sp.kron(sp.identity(N), sp.csr_matrix(Am.T), format="csr")
+
+
+# ---- operators of lifting models (reference linear/indexed.py, einsum.py:130-214) ----
+
+def indexedmult(B, P, x):
+    """IndexedMult: y[j, i, m] = -sum_l B[j, m, l] x[i, P[j, l]] with x (N, K),
+    B (J, M, L), P (J, L) -> y (J, N, M) (reference indexed.py:63-94, docstring :64; indexed.cu:27-48)."""
+    xg = x[:, P]  # (N, J, L): x[i, P[j, l]]
+    return -np.einsum("jml,ijl->jim", B, xg)
+
+
+def indexedmult_adj(B, P, x, K):
+    """IndexedMultAdj: y[i, P[j, l]] -= sum_m B[j, m, l] x[j, i, m] with
+    x (J, N, M) -> y (N, K) (reference indexed.py:34-61, docstring :35; indexed.cu:1-25)."""
+    J, N, M = x.shape
+    y = np.zeros((N, K))
+    contrib = np.einsum("jml,jim->jil", B, x)  # (J, N, L)
+    for j in range(J):
+        for l in range(P.shape[1]):
+            y[:, P[j, l]] -= contrib[j, :, l]
+    return y
+
+
+def tangled_matmul_r(A, x, trans=False):
+    """TangledMatrixMultR: y_mik = sum_jl x_jil A_jlmk, trans: A_mkjl
+    (reference einsum.py:130-173, docstring :131-132, the subscripts of its spmat :146)."""
+    return np.einsum("mkjl,jil->mik" if trans else "jlmk,jil->mik", A, x)
+
+
+def matmul_r_batched(A, x, trans=False):
+    """MatrixMultRBatched: y_jlk = sum_m x_jlm A_jmk, trans: A_jkm
+    (reference einsum.py:175-214, docstring :176-177, the subscripts of its spmat :187)."""
+    return np.einsum("jkm,jlm->jlk" if trans else "jmk,jlm->jlk", A, x)
+
+
+def indexedmult_matrix(B, P, K, N):
+    """IndexedMult as a sparse matrix on flat x (N, K) -> y (J, N, M), entry by
+    entry from the definition (duplicate indices in P add up)."""
+    J, M, L = B.shape
+    rows, cols, vals = [], [], []
+    for j in range(J):
+        for i in range(N):
+            for m in range(M):
+                for l in range(L):
+                    rows.append((j * N + i) * M + m)
+                    cols.append(i * K + P[j, l])
+                    vals.append(-B[j, m, l])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(J * N * M, N * K))
+
+
+def tangled_matmul_r_matrix(A, N, trans=False):
+    """TangledMatrixMultR as a sparse matrix on flat x (J, N, L) -> y (M, N, K),
+    entry by entry from the definition."""
+    if trans:
+        M, K, J, L = A.shape
+        a = lambda j, l, m, k: A[m, k, j, l]
+    else:
+        J, L, M, K = A.shape
+        a = lambda j, l, m, k: A[j, l, m, k]
+    rows, cols, vals = [], [], []
+    for m in range(M):
+        for i in range(N):
+            for k in range(K):
+                for j in range(J):
+                    for l in range(L):
+                        rows.append((m * N + i) * K + k)
+                        cols.append((j * N + i) * L + l)
+                        vals.append(a(j, l, m, k))
+    return sp.csr_matrix((vals, (rows, cols)), shape=(M * N * K, J * N * L))
+
+
+def matmul_r_batched_matrix(A, N, trans=False):
+    """MatrixMultRBatched as a sparse matrix on flat x (J, N, M) -> y (J, N, K),
+    entry by entry from the definition."""
+    if trans:
+        J, K, M = A.shape
+        a = lambda j, m, k: A[j, k, m]
+    else:
+        J, M, K = A.shape
+        a = lambda j, m, k: A[j, m, k]
+    rows, cols, vals = [], [], []
+    for j in range(J):
+        for i in range(N):
+            for k in range(K):
+                for m in range(M):
+                    rows.append((j * N + i) * K + k)
+                    cols.append((j * N + i) * M + m)
+                    vals.append(a(j, m, k))
+    return sp.csr_matrix((vals, (rows, cols)), shape=(J * N * K, J * N * M))

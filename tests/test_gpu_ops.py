@@ -280,3 +280,56 @@ def test_staged_host_arrays(gpu):
     y = op.y.new()
     op(x, y)
     np.testing.assert_allclose(y, ops.grad_matrix((6, 7), 2).dot(x), atol=1e-14)
+
+
+def _lifting_ops(N, seed=5):
+    """The operators and shapes of the reference's tests/einsum.py:8-20
+    (L = 50 labels, M = 60 regions, S = 2), with N points."""
+    from opymize_amd.linear import IndexedMultAdj, MatrixMultRBatched, TangledMatrixMultR
+    rng = np.random.default_rng(seed)
+    L, M, S = 50, 60, 2
+    A = rng.standard_normal((M, S, S))
+    B = rng.standard_normal((M, S, S + 1))
+    P = rng.integers(0, L, size=(M, S + 1))
+    P[3, 1] = P[3, 0]  # a repeated label in one region
+    tmp = rng.standard_normal((M, S + 1, 1, 1))
+    return rng, [
+        (IndexedMultAdj(L, N, P, B), ops.indexedmult_matrix(B, P, L, N).T),
+        (MatrixMultRBatched(N, A), ops.matmul_r_batched_matrix(A, N)),
+        (TangledMatrixMultR(N, tmp), ops.tangled_matmul_r_matrix(tmp, N)),
+    ]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("N", [64, 4096])
+def test_lifting_operators(gpu, dtype, N):
+    """IndexedMultAdj, MatrixMultRBatched and TangledMatrixMultR and their
+    adjoints against the oracle's entry-by-entry matrices, at the reference
+    test's shapes (N = 64*64 there): y (+)= A x, the adjoint, rowwise l^1."""
+    rng, cases = _lifting_ops(N)
+    for op, Mtx in cases:
+        for o, Mo in ((op, Mtx), (op.adjoint, Mtx.T.tocsr())):
+            assert (o.y.size, o.x.size) == Mo.shape
+            x = rng.standard_normal(o.x.size)
+            y0 = rng.standard_normal(o.y.size)
+            yd = dev(y0, dtype)
+            o(dev(x, dtype), yd, add=True)
+            close(host(yd), y0.astype(dtype) + Mo.dot(x.astype(dtype).astype(np.float64)), dtype, scale=10)
+            yd = dev(y0, dtype)
+            o(dev(x, dtype), yd)
+            close(host(yd), Mo.dot(x.astype(dtype).astype(np.float64)), dtype, scale=10)
+            if N == 64:
+                r = dev(np.zeros(o.y.size), dtype)
+                o.rowwise_lp(r)
+                close(host(r), ops.rowwise_lp(Mo, 1), dtype, scale=10)
+                r = dev(np.ones(o.y.size), dtype)
+                o.rowwise_lp(r, p=2, add=True)
+                close(host(r), 1.0 + ops.rowwise_lp(Mo, 2), dtype, scale=10)
+
+
+def test_indexedmult_refuses_bad_indices(gpu):
+    from opymize_amd.linear import IndexedMult
+    with pytest.raises(ValueError, match="indices"):
+        IndexedMult(4, 10, np.ones((2, 1, 3)), np.array([[0, 1, 4], [0, 0, 0]]))
+    with pytest.raises(ValueError):
+        IndexedMult(4, 10, np.ones((2, 1, 3)), np.zeros((2, 2), dtype=int))

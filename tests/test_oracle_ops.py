@@ -167,3 +167,57 @@ def test_rowwise_lp_against_explicit_rows():
     r = ops.rowwise_lp(G, 1)
     Gd = G.toarray()
     np.testing.assert_allclose(r, np.abs(Gd).sum(axis=1), atol=1e-12)
+
+
+# ---- lifting operators (reference linear/indexed.py, einsum.py:130-214) ----------------
+
+def _lifting_case(seed, J=4, M=2, L=3, K=6, N=5, dup=True):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((J, M, L))
+    P = rng.integers(0, K, (J, L))
+    if dup:
+        P[0, 2] = P[0, 0]  # a repeated label index: its coefficients add up
+    return B, P, K, N, rng
+
+
+def test_indexedmult_matches_its_definition_and_adjoint():
+    """einsum restatement = the matrix built entry by entry from the
+    docstring formula (indexed.py:64); the scatter restatement of the adjoint
+    (indexed.py:35) = that matrix's transpose."""
+    B, P, K, N, rng = _lifting_case(0)
+    Mx = ops.indexedmult_matrix(B, P, K, N)
+    x = rng.standard_normal((N, K))
+    np.testing.assert_allclose(ops.indexedmult(B, P, x).ravel(), Mx @ x.ravel(), atol=1e-14)
+    z = rng.standard_normal((B.shape[0], N, B.shape[1]))
+    np.testing.assert_allclose(ops.indexedmult_adj(B, P, z, K).ravel(), Mx.T @ z.ravel(), atol=1e-14)
+    # a hand example: one region, one output, labels 2 and 0 with weights 3 and -1
+    Bh = np.array([[[3.0, -1.0]]])
+    Ph = np.array([[2, 0]])
+    xh = np.array([[10.0, 20.0, 30.0]])
+    np.testing.assert_allclose(ops.indexedmult(Bh, Ph, xh), [[[-(3 * 30.0 - 10.0)]]])
+    np.testing.assert_allclose(ops.indexedmult_adj(Bh, Ph, np.array([[[2.0]]]), 3), [[2.0, 0.0, -6.0]])
+
+
+@pytest.mark.parametrize("trans", [False, True])
+def test_tangled_and_batched_match_their_definitions(trans):
+    rng = np.random.default_rng(1)
+    N = 7
+    A4 = rng.standard_normal((2, 3, 4, 5))
+    x = rng.standard_normal(((4, N, 5) if trans else (2, N, 3)))
+    Mt = ops.tangled_matmul_r_matrix(A4, N, trans)
+    np.testing.assert_allclose(ops.tangled_matmul_r(A4, x, trans).ravel(), Mt @ x.ravel(), atol=1e-13)
+    A3 = rng.standard_normal((3, 4, 2))
+    xb = rng.standard_normal(((3, N, 2) if trans else (3, N, 4)))
+    Mb = ops.matmul_r_batched_matrix(A3, N, trans)
+    np.testing.assert_allclose(ops.matmul_r_batched(A3, xb, trans).ravel(), Mb @ xb.ravel(), atol=1e-13)
+    # the transposed operator is the adjoint of the plain one
+    Mt0 = ops.tangled_matmul_r_matrix(A4, N, not trans)
+    np.testing.assert_allclose((Mt - Mt0.T).toarray(), 0.0, atol=0)
+    Mb0 = ops.matmul_r_batched_matrix(A3, N, not trans)
+    np.testing.assert_allclose((Mb - Mb0.T).toarray(), 0.0, atol=0)
+    # tangled = an ordinary product after grouping (i)(jl) x (jl)(mk) (einsum.py:133-135)
+    if not trans:
+        J, L, M, K = A4.shape
+        X = x.transpose(1, 0, 2).reshape(N, J * L)
+        Y = X @ A4.reshape(J * L, M * K)
+        np.testing.assert_allclose(ops.tangled_matmul_r(A4, x).transpose(1, 0, 2).reshape(N, M * K), Y, atol=1e-13)
