@@ -3,11 +3,15 @@
 Metric: solves/sec to a relative primal-dual gap of 1e-5, certified in FP64,
 for the batched fused PDHG on the learned-warm-start workload: TV-L2
 denoising of 1024x1024 images from the rectangle generator, lambda
-log-uniform in [1, 20], 256 instances per batch.  A step = solving one
-resident batch from its start point (cold, or the network's prediction with
---warm) until every instance is certified or the iteration cap.  The batched
-solver runs the accelerated steps of Chambolle & Pock (2011, Algorithm 2) on
-these strongly convex instances (--step-rule adaptive: the reference's rule).
+log-uniform in [1, 20], 256 instances per batch.  A step = the trained
+network's prediction of (x, y) for the resident batch, then the batched
+solver from that start until every instance is certified or the iteration
+cap, with a cold restart for any instance whose gap does not improve at the
+first check (--cold: from x = y = 0 instead).  The batched solver runs the
+accelerated steps of Chambolle & Pock (2011, Algorithm 2) on these strongly
+convex instances (--step-rule adaptive: the reference's rule).  After the
+timed steps the other start (cold, or warm with --cold) is solved once more
+for comparison.
 
 Printed next to it:
   * the plain single-instance GPU PDHG (the drop-in generic kernel chain of
@@ -20,7 +24,7 @@ Printed next to it:
     eager full-batch launches, timed with HIP events on the launch stream;
   * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--warm]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cold]
        python bench.py --roofline-only   (only the kernel-timing phase; for counter runs)
 """
 import argparse
@@ -212,7 +216,10 @@ def main():
     ap.add_argument("--multi", type=int, default=4, choices=[1, 2, 3, 4],
                     help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
-    ap.add_argument("--warm", action="store_true", help="start from the trained network's prediction")
+    ap.add_argument("--cold", action="store_true", help="start from x = y = 0 instead of the network's prediction")
+    ap.add_argument("--step-factor", type=float, default=4.0,
+                    help="initial tau/sigma split: tau = sqrt(bound * factor), sigma = sqrt(bound / factor)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the solve from the other start")
     ap.add_argument("--step-rule", default="accel", choices=["accel", "adaptive"],
                     help="step rule of the batched solver")
     ap.add_argument("--plain", type=int, default=6, help="instances for the plain single-instance baseline")
@@ -239,35 +246,50 @@ def main():
         log("roofline phase: %s" % ", ".join("%s %s" % kv for kv in kt.items()))
         return
 
-    warm = None
-    if args.warm:
-        from opymize_amd.warmstart.cnn import ConvNet, WarmStart
-        warm = WarmStart(ConvNet.load(os.path.join(ROOT, "opymize_amd", "warmstart", "weights_tv_l2.npz")))
+    from opymize_amd.warmstart.cnn import ConvNet, WarmStart
+    net = WarmStart(ConvNet.load(os.path.join(ROOT, "opymize_amd", "warmstart", "weights_tv_l2.npz")))
 
-    def one_step():
-        if warm is not None:
-            warm.predict_into(bat, lams)
+    def one_step(warm):
+        if warm:
+            net.predict_into(bat, lams)
         else:
             bat.set_start(None, None)
-        return bat.solve(steps=args.step_rule, step_bound=0.124, max_iters=args.max_iters,
-                         block_iters=args.block_iters, tol=1e-5, certify=True, fallback=warm is not None)
+        return bat.solve(steps=args.step_rule, step_bound=0.124, step_factor=args.step_factor,
+                         max_iters=args.max_iters, block_iters=args.block_iters, tol=1e-5, certify=True,
+                         fallback=warm)
+
+    warm = not args.cold
 
     for _ in range(args.warmup):
-        one_step()
+        one_step(warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     certified = 0
     iters = []
+    fallbacks = 0
     for _ in range(args.steps):
-        res = one_step()
+        res = one_step(warm)
+        fallbacks += len(bat.fallbacks)
         certified += sum(r["status"] == 1 for r in res)
         iters += [r["iter"] for r in res]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = certified / elapsed
     iters = np.array(iters)
-    log("batched: %d/%d certified in %.2fs, iterations min/median/max %d/%d/%d"
-        % (certified, args.steps * args.batch, elapsed, iters.min(), np.median(iters), iters.max()))
+    log("batched (%s start): %d/%d certified in %.2fs, iterations min/median/max %d/%d/%d"
+        % ("warm" if warm else "cold", certified, args.steps * args.batch, elapsed, iters.min(), np.median(iters),
+           iters.max()))
+    other = None
+    if not args.no_compare:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = one_step(not warm)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        c = int(sum(r["status"] == 1 for r in res))
+        other = {"start": "cold" if warm else "warm", "certified": c, "seconds": dt, "solves_per_sec": c / dt,
+                 "iterations_median": float(np.median([r["iter"] for r in res]))}
+        log("batched (%s start, for comparison): %d/%d certified in %.2fs" % (other["start"], c, args.batch, dt))
 
     kt = roofline_phase(bat, args.reps, args.step_rule)
     variant = "adaptive" if args.step_rule == "adaptive" else "accelerated steps"
@@ -308,7 +330,10 @@ def main():
                    "step_rule": {"accel": "accelerated (Chambolle-Pock Alg. 2, gamma = lambda)",
                                  "adaptive": "adaptive (reference rule)"}[args.step_rule],
                    "plain_step_rule": "adaptive (reference rule)",
-                   "warm_start": "cnn (6x3x3 conv, 32 ch, bf16 MFMA) with cold-start fallback" if warm else "none",
+                   "warm_start": ("cnn (6-layer 3x3 conv, 32 channels, bf16 MFMA), cold restart where the gap "
+                                  "does not improve (%d instances)" % fallbacks) if warm else "none (cold start)",
+                   "step_factor": args.step_factor,
+                   "other_start": other,
                    "tile": list(args.tile), "block_iters": args.block_iters, "iterations_per_launch": k_launch,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
                    "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max()),

@@ -348,7 +348,7 @@ int opy_tv_iter_f64(const double* x_in, double* x_out, const double* y_in, doubl
  * adaptive steps, whose rule needs every iteration's residuals): the same iterates as k
  * opy_tv_iter launches with steps SCALAR / PRECOND / ACCEL (PDHG.iteration_step, pdhg.py:124-153,
  * k times).  Each block loads its tile and k pixels around it into LDS and iterates there.  The
- * step state is read, not advanced: with OPY_TV_STEPS_ACCEL call opy_tv_accel k times after.
+ * step state is read, not advanced: with OPY_TV_STEPS_ACCEL call opy_tv_accel_n with count k after.
  * Fails with an error when the region of a tile (3*C values per pixel) exceeds 64 KiB of LDS. */
 int opy_tv_multi_f32(const float* x_in, float* x_out, const float* y_in, float* y_out, const float* f,
                      const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
@@ -364,6 +364,9 @@ int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy
 /* accelerated step update of the running instances after an iteration (OPY_TV_STEPS_ACCEL) */
 int opy_tv_accel(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks, int B,
                  void* stream);
+/* the same for `count` consecutive iterations in one launch (after opy_tv_multi_* with k = count) */
+int opy_tv_accel_n(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks, int B,
+                   int count, void* stream);
 /* FP64 gap partials per tile (reference pdhg.py:23-44) */
 int opy_tv_gap_f32(const float* x, const float* y, const float* f, const opy_tv_inst_t* insts,
                    const int32_t* active_inst, const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg,
@@ -392,6 +395,11 @@ int opy_tv_compact(const opy_tv_inst_t* insts, const opy_tv_check_t* checks, int
 /* y = act(conv(x, W) + b); wt from opy_conv3x3_prep_weights (9*32*32 bf16); act 0 none, 1 ReLU */
 int opy_conv3x3_fwd_bf16(const float* x, float* y, const void* wt, const float* bias, int N, int H, int W, int act,
                          int grid, void* stream);
+/* the same with the activations' element type chosen per side: x_bf16 / y_bf16 = 1 for BF16 NHWC (the
+ * operands are rounded to BF16 for the matrix cores in any case, so BF16 activations between layers give the
+ * same results at half the traffic), 0 for float32 */
+int opy_conv3x3_fwd_io(const void* x, void* y, const void* wt, const float* bias, int N, int H, int W, int act,
+                       int x_bf16, int y_bf16, int grid, void* stream);
 /* per-block partials (grid x (9*32*32 + 32) floats) of dW[tap][co][ci] and dbias[co] */
 int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,
                            void* stream);

@@ -31,8 +31,12 @@ __device__ inline __bf16 to_bf16(float v) { return (__bf16)v; }
 
 // ---- forward / backward-data -------------------------------------------------
 // y[n,h,w,co] = act(b[co] + sum_{tap,ci} W[tap][co][ci] * x[n, h+dy-1, w+dx-1, ci])
-// act: 0 none, 1 ReLU.  bias may be NULL.
-__global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const float* __restrict__ x, float* __restrict__ y,
+// act: 0 none, 1 ReLU.  bias may be NULL.  TX / TY: element type of x / y in
+// HBM (float, or __bf16 for activations between layers: the operands are
+// rounded to BF16 for the matrix cores anyway, so a BF16 copy of a layer's
+// output gives the next layer the same operands at half the traffic).
+template <typename TX, typename TY>
+__global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__ x, TY* __restrict__ y,
                                                           const __bf16* __restrict__ wt, const float* __restrict__ bias,
                                                           int N, int H, int W, int act) {
   __shared__ __align__(16) __bf16 xs[kHalo * kHalo * kPixStride];
@@ -51,18 +55,30 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const float* __restric
     const int rem = t - n * tx * ty;
     const int h0 = (rem / tx) * kTile, w0 = (rem % tx) * kTile;
     __syncthreads();  // previous tile done with xs (and ws staged)
-    const float* xb = x + (int64_t)n * H * W * kCh;
-    for (int e = tid; e < kHalo * kHalo * (kCh / 4); e += 256) {
-      const int q = e % (kCh / 4), p = e / (kCh / 4);
-      const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (hh >= 0 && hh < H && ww >= 0 && ww < W)
-        v = *reinterpret_cast<const float4*>(xb + ((int64_t)hh * W + ww) * kCh + 4 * q);
-      __bf16* d = xs + p * kPixStride + 4 * q;
-      d[0] = to_bf16(v.x);
-      d[1] = to_bf16(v.y);
-      d[2] = to_bf16(v.z);
-      d[3] = to_bf16(v.w);
+    const TX* xb = x + (int64_t)n * H * W * kCh;
+    if constexpr (sizeof(TX) == 4) {
+      for (int e = tid; e < kHalo * kHalo * (kCh / 4); e += 256) {
+        const int q = e % (kCh / 4), p = e / (kCh / 4);
+        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+          v = *reinterpret_cast<const float4*>(xb + ((int64_t)hh * W + ww) * kCh + 4 * q);
+        __bf16* d = xs + p * kPixStride + 4 * q;
+        d[0] = to_bf16(v.x);
+        d[1] = to_bf16(v.y);
+        d[2] = to_bf16(v.z);
+        d[3] = to_bf16(v.w);
+      }
+    } else {
+      // 8 BF16 channels (16 B) per access
+      for (int e = tid; e < kHalo * kHalo * (kCh / 8); e += 256) {
+        const int q = e % (kCh / 8), p = e / (kCh / 8);
+        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+        bf16x8 v = {};
+        if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+          v = *reinterpret_cast<const bf16x8*>(xb + ((int64_t)hh * W + ww) * kCh + 8 * q);
+        *reinterpret_cast<bf16x8*>(xs + p * kPixStride + 8 * q) = v;
+      }
     }
     __syncthreads();
     f32x4 acc[4][2];
@@ -86,7 +102,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const float* __restric
         for (int j = 0; j < 2; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
       }
     }
-    float* yb = y + (int64_t)n * H * W * kCh;
+    TY* yb = y + (int64_t)n * H * W * kCh;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const int h = h0 + wave * 4 + m;
@@ -101,7 +117,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const float* __restric
           if (w >= W) continue;
           float v = acc[m][j][r] + bv;
           if (act == 1) v = v > 0.f ? v : 0.f;
-          yb[((int64_t)h * W + w) * kCh + co] = v;
+          yb[((int64_t)h * W + w) * kCh + co] = (TY)v;
         }
       }
     }
@@ -297,9 +313,30 @@ OPY_API int opy_conv3x3_fwd_bf16(const float* x, float* y, const void* wt, const
   if (!x || !y || !wt || N < 1 || H < 1 || W < 1 || grid < 1) return fail(kInvalid, "conv3x3_fwd: bad arguments");
   if (x == y) return fail(kInvalid, "conv3x3_fwd: in-place convolution is not supported");
   if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_fwd: input must be 16-byte aligned");
-  hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3(grid), dim3(256), 0, as_stream(s), x, y,
+  hipLaunchKernelGGL((conv3x3_fwd_kernel<float, float>), dim3(grid), dim3(256), 0, as_stream(s), x, y,
                      static_cast<const __bf16*>(wt), bias, N, H, W, act);
   return check_launch("conv3x3_fwd");
+}
+OPY_API int opy_conv3x3_fwd_io(const void* x, void* y, const void* wt, const float* bias, int N, int H, int W,
+                               int act, int x_bf16, int y_bf16, int grid, void* s) {
+  if (!x || !y || !wt || N < 1 || H < 1 || W < 1 || grid < 1) return fail(kInvalid, "conv3x3_fwd_io: bad arguments");
+  if (x == y) return fail(kInvalid, "conv3x3_fwd_io: in-place convolution is not supported");
+  if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_fwd_io: input must be 16-byte aligned");
+  const __bf16* w = static_cast<const __bf16*>(wt);
+  hipStream_t st = as_stream(s);
+  if (x_bf16 && y_bf16)
+    hipLaunchKernelGGL((conv3x3_fwd_kernel<__bf16, __bf16>), dim3(grid), dim3(256), 0, st,
+                       static_cast<const __bf16*>(x), static_cast<__bf16*>(y), w, bias, N, H, W, act);
+  else if (x_bf16)
+    hipLaunchKernelGGL((conv3x3_fwd_kernel<__bf16, float>), dim3(grid), dim3(256), 0, st,
+                       static_cast<const __bf16*>(x), static_cast<float*>(y), w, bias, N, H, W, act);
+  else if (y_bf16)
+    hipLaunchKernelGGL((conv3x3_fwd_kernel<float, __bf16>), dim3(grid), dim3(256), 0, st,
+                       static_cast<const float*>(x), static_cast<__bf16*>(y), w, bias, N, H, W, act);
+  else
+    hipLaunchKernelGGL((conv3x3_fwd_kernel<float, float>), dim3(grid), dim3(256), 0, st,
+                       static_cast<const float*>(x), static_cast<float*>(y), w, bias, N, H, W, act);
+  return check_launch("conv3x3_fwd_io");
 }
 
 OPY_API int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,

@@ -52,15 +52,21 @@ __global__ void __launch_bounds__(kTB) tv_adapt_kernel(const opy_tv_inst_t* __re
 }
 
 // ---- accelerated step update (Chambolle & Pock 2011, Algorithm 2) -----------
-// tau <- theta tau, sigma <- sigma / theta with the theta the iteration used
+// tau <- theta tau, sigma <- sigma / theta with the theta each iteration used,
+// for `count` consecutive iterations (the same operations as count updates)
 __global__ void __launch_bounds__(kTB) tv_accel_kernel(const opy_tv_inst_t* __restrict__ insts,
                                                        opy_adapt_state_t* __restrict__ st,
-                                                       const opy_tv_check_t* __restrict__ chk, int B) {
+                                                       const opy_tv_check_t* __restrict__ chk, int B, int count) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B || chk[b].status != 0) return;
-  const double th = accel_theta(OPY_TV_STEPS_ACCEL, insts[b], st[b].tau);
-  st[b].tau *= th;
-  st[b].sigma /= th;
+  double tau = st[b].tau, sigma = st[b].sigma;
+  for (int c = 0; c < count; ++c) {
+    const double th = accel_theta(OPY_TV_STEPS_ACCEL, insts[b], tau);
+    tau *= th;
+    sigma /= th;
+  }
+  st[b].tau = tau;
+  st[b].sigma = sigma;
 }
 
 // ---- primal-dual gap (reference pdhg.py:23-44) ------------------------------
@@ -295,8 +301,15 @@ using namespace opy;
 OPY_API int opy_tv_accel(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk, int B,
                          void* s) {
   if (!insts || !st || !chk || B < 1) return fail(kInvalid, "tv_accel: bad arguments");
-  hipLaunchKernelGGL(tv_accel_kernel, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, as_stream(s), insts, st, chk, B);
+  hipLaunchKernelGGL(tv_accel_kernel, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, as_stream(s), insts, st, chk, B, 1);
   return check_launch("tv_accel");
+}
+OPY_API int opy_tv_accel_n(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk, int B,
+                           int count, void* s) {
+  if (!insts || !st || !chk || B < 1 || count < 1) return fail(kInvalid, "tv_accel_n: bad arguments");
+  hipLaunchKernelGGL(tv_accel_kernel, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, as_stream(s), insts, st, chk, B,
+                     count);
+  return check_launch("tv_accel_n");
 }
 OPY_API int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk,
                          const double* partials, int B, int C, const opy_adapt_params_t* p, void* s) {

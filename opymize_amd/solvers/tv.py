@@ -64,6 +64,7 @@ _SIGS = {
 _PLAIN = {
     "opy_tv_adapt": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
     "opy_tv_accel": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_accel_n": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_check": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_compact": [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int,
@@ -328,8 +329,7 @@ class TVBatch(object):
                    P(self.f), P(self.insts), P(self.state), P(self.active_inst), P(self.act_off), P(self.nact),
                    ctypes.byref(cfg), int(k), self.grid, s)
         if cfg.steps == STEP_MODES["accel"]:
-            for _ in range(k):
-                self._call("opy_tv_accel", P(self.insts), P(self.state), P(self.checks), self.B, s)
+            self._call("opy_tv_accel_n", P(self.insts), P(self.state), P(self.checks), self.B, int(k), s)
 
     def _iterations(self, n, cfg, adapt, params):
         """n iterations starting from buffer self.cur (updates self.cur)."""

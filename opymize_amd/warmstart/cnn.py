@@ -30,6 +30,7 @@ def _bind():
     P, I, L, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
     sigs = {
         "opy_conv3x3_fwd_bf16": [P, P, P, P, I, I, I, I, I, P],
+        "opy_conv3x3_fwd_io": [P, P, P, P, I, I, I, I, I, I, I, P],
         "opy_conv3x3_wgrad_bf16": [P, P, P, I, I, I, I, P],
         "opy_sum_partials_f32": [P, I, I, P, P],
         "opy_conv3x3_prep_weights": [P, P, I, P],
@@ -80,6 +81,7 @@ class ConvNet(object):
         self._wb = [torch.empty(9 * CH * CH, dtype=torch.bfloat16, device="cuda") for _ in range(layers)]
         self.grid = 8 * torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
         self._dirty = True
+        self._ibufs = {}  # inference buffers by input shape: two BF16 activations and the float32 output
 
     # -- parameters -------------------------------------------------------
     def W(self, l):
@@ -115,13 +117,33 @@ class ConvNet(object):
         return net
 
     # -- passes ----------------------------------------------------------------
-    def forward(self, x, keep=False, bufs=None):
+    def forward(self, x, keep=False, bufs=None, bf16_acts=False):
         """x: (N, H, W, 32) float32 device tensor.  Returns the output (and the
-        list of activations when keep=True, needed by backward)."""
+        list of activations when keep=True, needed by backward).
+
+        bf16_acts=True (inference): the activations between layers are kept in
+        BF16 (the layers round their operands to BF16 in any case, so the
+        output is the same) in buffers owned by the network; the output is
+        float32 and valid until the next such call."""
         self._prep()
         N, H, W, C = x.shape
         assert C == CH and x.dtype == torch.float32 and x.is_contiguous()
         s = _hip.stream()
+        if bf16_acts and not keep:
+            key = tuple(x.shape)
+            if key not in self._ibufs:
+                self._ibufs = {key: (torch.empty(key, dtype=torch.bfloat16, device="cuda"),
+                                     torch.empty(key, dtype=torch.bfloat16, device="cuda"),
+                                     torch.empty(key, dtype=torch.float32, device="cuda"))}
+            b16a, b16b, outf = self._ibufs[key]
+            cur, cur16 = x, 0
+            for l in range(self.layers):
+                last = l == self.layers - 1
+                out = outf if last else (b16a, b16b)[l % 2]
+                _call("opy_conv3x3_fwd_io", _hip.ptr(cur), _hip.ptr(out), _hip.ptr(self._wf[l]), _hip.ptr(self.b(l)),
+                      N, H, W, 0 if last else 1, cur16, 0 if last else 1, self.grid, s)
+                cur, cur16 = out, 1
+            return outf
         acts = [x]
         cur = x
         for l in range(self.layers):
@@ -209,6 +231,11 @@ def encode_targets(images, xs, ys):
     return out
 
 
+def _rows(t, off, stride, n, length):
+    """n segments of `length` elements of the flat tensor t, `stride` apart from `off`, as an (n, length) view."""
+    return t.as_strided((n, length), (stride, 1), storage_offset=t.storage_offset() + off)
+
+
 class WarmStart(object):
     """Predicts packed (x0, y0) start vectors for a TVBatch of C=1 instances."""
 
@@ -216,33 +243,59 @@ class WarmStart(object):
         self.net = net
         self.tvw = tvw
         self.chunk = chunk
+        self._inp = {}  # input tensors by shape: channels 2.. stay zero, 0 and 1 are rewritten per chunk
 
     def predict_into(self, batch, lams):
-        """Writes the prediction into batch.x[0] / batch.y[0] (the start buffers)."""
+        """Writes the prediction into batch.x[0] / batch.y[0] (the start buffers).
+
+        Instances of one shape go through the network in chunks; a chunk whose
+        instances sit at equal spacing in the packed buffers (consecutive
+        instances of one shape) is read and written through one strided view
+        per array, others instance by instance."""
         assert batch.C == 1
         f_all = batch.f
         groups = {}
         for b, shp in enumerate(batch.shapes):
             groups.setdefault(tuple(shp), []).append(b)
-        bufs = None
         for (H, W), idx in groups.items():
+            HW = H * W
             for k in range(0, len(idx), self.chunk):
                 sel = idx[k:k + self.chunk]
                 n = len(sel)
-                inp = torch.zeros((n, H, W, CH), dtype=torch.float32, device="cuda")
-                for j, b in enumerate(sel):
-                    off = batch.offs[b]
-                    inp[j, ..., 0] = f_all[off:off + H * W].view(H, W).float() - 0.5
-                    inp[j, ..., 1] = float(np.log(lams[b]) / 3.0)
-                if bufs is None or bufs[0].shape != inp.shape:
-                    bufs = [torch.empty_like(inp), torch.empty_like(inp)]
-                out = self.net.forward(inp, bufs=bufs)
-                for j, b in enumerate(sel):
-                    off = batch.offs[b]
-                    xv = inp[j, ..., 0] + 0.5 + out[j, ..., 0]
-                    batch.x[0][off:off + H * W].copy_(xv.reshape(-1).to(batch.dtype))
-                    ydst = batch.y[0][2 * off:2 * off + 2 * H * W]
-                    ydst.copy_(out[j, ..., 1:3].reshape(-1).to(batch.dtype))
+                offs = [batch.offs[b] for b in sel]
+                steps = {o1 - o0 for o0, o1 in zip(offs, offs[1:])}
+                stride = steps.pop() if len(steps) == 1 else (HW if n == 1 else None)
+                inp = self._inp.get((n, H, W))
+                if inp is None:
+                    inp = self._inp[(n, H, W)] = torch.zeros((n, H, W, CH), dtype=torch.float32, device="cuda")
+                loglam = torch.as_tensor(np.log(np.asarray([lams[b] for b in sel], dtype=np.float64)) / 3.0,
+                                         dtype=torch.float32).cuda()
+                inp[..., 1] = loglam[:, None, None]
+                if stride is not None:
+                    fv = _rows(f_all, offs[0], stride, n, HW)
+                    inp[..., 0] = fv.view(n, H, W).float() - 0.5
+                else:
+                    for j, off in enumerate(offs):
+                        inp[j, ..., 0] = f_all[off:off + HW].view(H, W).float() - 0.5
+                out = self.net.forward(inp, bf16_acts=True)
+                xv = (inp[..., 0] + 0.5 + out[..., 0]).reshape(n, HW).to(batch.dtype)
+                yv = out[..., 1:3].reshape(n, 2 * HW).to(batch.dtype)
+                if stride is not None:
+                    _rows(batch.x[0], offs[0], stride, n, HW).copy_(xv)
+                    ydst = _rows(batch.y[0], 2 * offs[0], 2 * stride, n, 2 * HW)
+                    ydst.copy_(yv)
                     # feasible dual start: project every pixel's pair onto |y| <= tvw
-                    _hip.call("l1norms_proj", batch.dtype, _hip.ptr(ydst), H * W, 1, 2, float(self.tvw), 0,
-                              _hip.stream())
+                    if stride == HW:
+                        _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * offs[0]:2 * offs[0] + 2 * n * HW]),
+                                  n * HW, 1, 2, float(self.tvw), 0, _hip.stream())
+                    else:
+                        for off in offs:
+                            _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * off:2 * off + 2 * HW]), HW,
+                                      1, 2, float(self.tvw), 0, _hip.stream())
+                else:
+                    for j, off in enumerate(offs):
+                        batch.x[0][off:off + HW].copy_(xv[j])
+                        ydst = batch.y[0][2 * off:2 * off + 2 * HW]
+                        ydst.copy_(yv[j])
+                        _hip.call("l1norms_proj", batch.dtype, _hip.ptr(ydst), HW, 1, 2, float(self.tvw), 0,
+                                  _hip.stream())

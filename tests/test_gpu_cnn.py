@@ -154,3 +154,32 @@ def test_trained_network_gain_over_cold_start(gpu):
     for b, lam in enumerate(lams):
         bound = np.sqrt(2 * max(gap_w[b], 0) / lam) + np.sqrt(2 * max(gap_c[b], 0) / lam)
         assert np.linalg.norm(x_w[b] - x_c[b]) <= bound * 1.001 + 1e-3
+
+
+@pytest.mark.parametrize("shape", [(2, 21, 37), (1, 64, 48)])
+def test_bf16_activations_give_the_same_output(gpu, shape):
+    """Inference with BF16 activations between layers returns bitwise the
+    output of the float32-activation forward pass: every layer rounds its
+    operands to BF16 for the matrix cores, so storing them as BF16 changes no
+    operand.  Also the float32 -> float32 mode of opy_conv3x3_fwd_io equals
+    opy_conv3x3_fwd_bf16."""
+    import os
+    from opymize_amd.warmstart.cnn import ConvNet, _call
+    from opymize_amd import _hip
+    N, H, W = shape
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    net = ConvNet.load(os.path.join(root, "opymize_amd", "warmstart", "weights_tv_l2.npz"))
+    rng = np.random.default_rng(4)
+    x = np.zeros((N, H, W, 32), dtype=np.float32)
+    x[..., 0] = rng.random((N, H, W)) - 0.5
+    x[..., 1] = 0.7
+    xd = torch.as_tensor(x).cuda()
+    ref = net.forward(xd).cpu().numpy()
+    out = net.forward(xd, bf16_acts=True).cpu().numpy()
+    np.testing.assert_array_equal(out, ref)
+    y1, y2 = torch.empty_like(xd), torch.empty_like(xd)
+    _call("opy_conv3x3_fwd_bf16", _hip.ptr(xd), _hip.ptr(y1), _hip.ptr(net._wf[0]), _hip.ptr(net.b(0)), N, H, W, 1,
+          net.grid, _hip.stream())
+    _call("opy_conv3x3_fwd_io", _hip.ptr(xd), _hip.ptr(y2), _hip.ptr(net._wf[0]), _hip.ptr(net.b(0)), N, H, W, 1,
+          0, 0, net.grid, _hip.stream())
+    np.testing.assert_array_equal(y1.cpu().numpy(), y2.cpu().numpy())
