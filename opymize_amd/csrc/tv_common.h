@@ -1,5 +1,5 @@
 // Shared helpers of the fused TV kernels (tv_passes.hip, tv_iter.hip,
-// tv_control.hip): configuration, tile map, vector loads and streaming
+// tv_multi.hip, tv_control.hip): configuration, tile map, vector loads and streaming
 // stores, per-pixel stencils, cross-lane moves and the host-side launch
 // helpers.  Everything sits in an unnamed namespace: each translation
 // unit gets its own copy.
@@ -14,31 +14,35 @@
 // i.e. PDHG(G, L1Norms(N, (C, 2), tvw), GradientOp(scheme="forward")) of the
 // reference (pdhg.py:124-187, diff.cu:64-71, :114-123, proj.cu:34-44).
 //
-// One iteration is two passes over HBM:
+// An iteration is one pass over HBM (tv_iter.hip) or two (tv_passes.hip), or
+// K iterations share one pass (tv_multi.hip); the two passes are:
 //   pass 1 (primal): x+ = prox_{tau G}(x - tau * A^T y)           reads x, y, f; writes x+
 //   pass 2 (dual):   y+ = proj_{|.|<=tvw}(y + sigma * A(2x+ - x))  reads x, x+, y; writes y+
 // The over-relaxed A x_bar = 2 A x+ - A x of the reference is applied as
 // A(2x+ - x) (equal by linearity).  With adaptive steps pass 2 also forms the
 // four residual norms of pdhg.py:155-164.
 //
-// Thread layout: a tile is tw columns x (256/tw)*RPT rows; a thread owns one
+// Thread layout of the two-pass kernels: a tile is tw columns x (256/tw)*RPT
+// rows (the single-pass and multi-iteration kernels use tw = 64); a thread owns one
 // column segment of RPT consecutive rows and marches down it, carrying the
 // row above / below in registers.  Horizontal neighbours come from the
 // neighbouring lane by cross-lane shuffles; only the first/last lane of a row
 // segment of a wavefront loads (or, for y+, recomputes) its neighbour.
 //
-// A persistent grid walks the tiles of the still-running instances:
-// active_inst / act_off hold the running instances and the exclusive prefix
-// of their tile counts; block k takes tiles k, k + grid, k + 2 grid, ... so
-// that the blocks running at one time work on neighbouring tiles (DRAM page
-// locality, shared halos; contiguous chunks per block measured slower).
+// The kernels walk the tiles of the still-running instances with a grid
+// stride: active_inst / act_off hold the running instances and the exclusive
+// prefix of their tile counts, and block k takes tiles k, k + grid, ...  The
+// solver launches one block per running tile (rounded up), so the hardware
+// dispatcher keeps the tiles in flight contiguous in memory (DRAM page
+// locality, halos shared through L2); a persistent grid of resident blocks,
+// whose blocks drift apart, measured 14% slower with 1.34x the reads.
 // Blocks are renumbered so that consecutive tiles run on the same XCD (the
-// dispatcher deals blocks round-robin to the 8 XCDs, each with its own L2),
-// which keeps the halo rows between tiles in one L2.
-// The single-pass kernel (tv_iter_kernel) forms its halos "transposed", see
-// its comment.  All
-// per-instance reductions (residuals, gap) go through per-tile FP64 partials
-// that a per-instance kernel folds in a fixed order (reproducible results).
+// dispatcher deals blocks round-robin to the 8 XCDs, each with its own L2).
+// The single-pass kernel (tv_iter_kernel) forms its halos "transposed", and
+// the multi-iteration kernel (tv_multi_kernel) stages a tile with a K-pixel
+// halo in LDS; see their comments.  All per-instance reductions (residuals,
+// gap) go through per-tile FP64 partials that a per-instance kernel folds in
+// a fixed order (reproducible results).
 #pragma once
 
 #include "opy_common.h"

@@ -193,11 +193,16 @@ class TVBatch(object):
         self.act_off = torch.zeros(self.B + 1, dtype=torch.int32, device="cuda")
         self.nact = torch.zeros(1, dtype=torch.int32, device="cuda")
         self.iter_counter = torch.zeros(1, dtype=torch.int64, device="cuda")
-        # grid=0 makes the library size a persistent launch to the blocks
-        # resident at once; by default (no override) every launch gets one
-        # block per tile of the running instances, rounded up to a bucket
-        # (see _launch_grid), which keeps the tiles in flight contiguous
-        self.grid = int(max(1, min(self._grid_override, t0))) if self._grid_override is not None else t0
+        # by default every launch gets one block per tile of the running
+        # instances, rounded up to a bucket (see _launch_grid), which keeps the
+        # tiles in flight contiguous; grid=0 makes the library size a
+        # persistent launch to the blocks resident at once, grid=n > 0 fixes n
+        if self._grid_override is None:
+            self.grid = t0
+        elif self._grid_override == 0:
+            self.grid = 0
+        else:
+            self.grid = int(min(self._grid_override, t0))
         self.graph = None
 
     def _launch_grid(self, tiles):

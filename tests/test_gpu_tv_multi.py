@@ -175,3 +175,25 @@ def test_autotune_considers_iterations_per_launch(gpu):
                                 term_pd_gap=1e-30)
         assert np.abs(xs[b] - xo).max() < 1e-9, b
     autotune._cache.clear()
+
+
+@pytest.mark.parametrize("multi,rpt", [(1, 8), (4, 4)])
+def test_launch_shapes_give_the_same_iterates(gpu, multi, rpt):
+    """One block per running tile (the default), the persistent grid of
+    resident blocks (grid=0) and a small fixed grid (each block walks many
+    tiles) give bitwise the same iterates and gaps."""
+    from opymize_amd.solvers.tv import TVBatch
+    imgs = [rects(H, W, 40 + i) for i, (H, W) in enumerate(SHAPES)]
+    lams = [2.0, 7.5, 1.0, 15.0, 4.0, 3.3]
+    out = {}
+    for grid in (None, 0, 7):
+        bat = TVBatch(imgs, lams, "l2", precision="float", tile=(64, rpt), grid=grid, multi=multi)
+        if grid == 0:
+            assert bat.grid == 0
+        res = bat.solve(steps="accel", step_bound=0.124, max_iters=48, block_iters=24, tol=1e-30)
+        xs, ys = bat.solution()
+        out[grid] = (xs, ys, [r["gap"] for r in res])
+    for g in (0, 7):
+        for a, b in zip(out[None][0] + out[None][1], out[g][0] + out[g][1]):
+            np.testing.assert_array_equal(a, b)
+        assert out[None][2] == out[g][2]
