@@ -155,6 +155,15 @@ struct PrimalStep {
 #pragma unroll
     for (int k = 0; k < C; ++k) coef(PC ? precond_tau<T>(1, 1, 3, 3, k, cfg) : (T)tau_s, tau[k], a[k], b[k]);
   }
+  // scalar steps from coefficients formed elsewhere by coef()
+  __device__ PrimalStep(T t, T ak, T bk, const opy_tv_inst_t& I) : lam(I.lam), data(I.data) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      tau[k] = t;
+      a[k] = ak;
+      b[k] = bk;
+    }
+  }
   __device__ void coef(T t, T& tk, T& ak, T& bk) const {
     const double tl = (double)t * lam;
     tk = t;

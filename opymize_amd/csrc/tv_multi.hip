@@ -46,7 +46,9 @@ constexpr int kMultiLdsMax = 64 * 1024;
 //
 // Steps: tau, sigma (and theta with accelerated steps) of iteration i are
 // derived from the instance's state exactly as the single-step launches and
-// opy_tv_accel would advance it; the state itself is not written.
+// opy_tv_accel would advance it, by one thread per tile into a small LDS
+// table (this FP64 work was a fifth of the instruction issue when every wave
+// did it); the state itself is not written.
 template <typename T, int C, int RPT, int K, bool PC>
 __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ xin, T* __restrict__ xout,
                                                        const T* __restrict__ yin, T* __restrict__ yout,
@@ -67,6 +69,7 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
   static_assert(multi_lds_bytes<T, C, RPT, K>() <= kMultiLdsMax, "LDS budget");
   __shared__ T sy[2 * C][NP];
   __shared__ T sxb[C][NP];
+  __shared__ T sstep[K][5];  // scalar steps per iteration: tau, a, b (prox coefficients), theta, dual step
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
@@ -124,15 +127,33 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
       if (j < NR || srow(j) >= 0)
 #pragma unroll
         for (int e = 0; e < 2 * C; ++e) sy[e][sidx(j)] = yv[j][e];
+    // the K iterations' scalar steps, formed once per tile (FP64, as K single
+    // launches and opy_tv_accel would form them) instead of in every wave
+    if (!PC && threadIdx.x == 0) {
+      double tau_d = st[tr.b].tau, sig_d = st[tr.b].sigma;
+      for (int i = 0; i < K; ++i) {
+        const double th_d = accel_theta(cfg.steps, I, tau_d);
+        const PrimalStep<T, C, false> p(tau_d, I, cfg);
+        sstep[i][0] = p.tau[0];
+        sstep[i][1] = p.a[0];
+        sstep[i][2] = p.b[0];
+        sstep[i][3] = (T)th_d;
+        sstep[i][4] = (T)(sig_d / th_d);
+        tau_d *= th_d;
+        sig_d /= th_d;
+      }
+    }
     __syncthreads();
-    double tau_d = st[tr.b].tau, sig_d = st[tr.b].sigma;
     const T tvw = (T)I.tvw;
+    // preconditioned steps do not change between iterations (theta = 1)
+    const PrimalStep<T, C, PC> prox_pc(0.0, I, cfg);
+    const DualStep<T, C, PC> ds_pc(T(0), cfg);
 #pragma unroll 1
     for (int i = 1; i <= K; ++i) {
-      const double th_d = accel_theta(cfg.steps, I, tau_d);
-      const T theta = (T)th_d;
-      const PrimalStep<T, C, PC> prox(tau_d, I, cfg);
-      const DualStep<T, C, PC> ds((T)(sig_d / th_d), cfg);
+      const T* sp = sstep[i - 1];
+      const T theta = PC ? T(1) : sp[3];
+      const PrimalStep<T, C, PC> prox = PC ? prox_pc : PrimalStep<T, C, PC>(sp[0], sp[1], sp[2], I);
+      const DualStep<T, C, PC> ds = PC ? ds_pc : DualStep<T, C, PC>(sp[4], cfg);
       // one iteration; L1: data kind, M: boundary masks (both uniform over the tile)
       auto iterate = [&](auto l1_tag, auto m_tag) {
         constexpr bool L1 = decltype(l1_tag)::value, M = decltype(m_tag)::value;
@@ -188,8 +209,6 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
         if (inner) iterate(FF{}, FF{});
         else iterate(FF{}, TT{});
       }
-      tau_d *= th_d;
-      sig_d /= th_d;
     }
     // ---- stores: the tile pixels (tile columns only)
 #pragma unroll
