@@ -342,6 +342,91 @@ __device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T
   }
 }
 
+// Scalar steps of one iteration (constant and accelerated step rules) in the
+// form the folded stencils below use: the prox coefficients a, b of
+// PrimalStep (from tau rounded to T), theta, and the step sizes folded into
+// the difference weights, p = tau w_k / h and q = sigma w_k / h per axis,
+// each rounded once from FP64.  Then x - tau A^T y costs one subtraction and
+// one fused multiply-add per difference, and so does y + sigma A xbar, where
+// PrimalStep/DualStep with adjoint_px/grad_px spend a further product on the
+// weight and one on the step (preconditioned steps vary per pixel and
+// adaptive steps need A^T y and A xbar for their residuals, so those keep
+// the unfolded form).
+template <typename T, int C>
+struct FoldedStep {
+  T a, b, theta;
+  T p[2][C], q[2][C];
+};
+
+// tau, sigma: this iteration's primal and dual steps (sigma already divided by theta)
+template <typename T, int C>
+__device__ inline FoldedStep<T, C> fold_step(double tau, double sigma, double theta, const opy_tv_inst_t& I,
+                                             const Cfg& cfg) {
+  FoldedStep<T, C> s;
+  const PrimalStep<T, C, false> ps(tau, I, cfg);
+  s.a = ps.a[0];
+  s.b = ps.b[0];
+  s.theta = (T)theta;
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    s.p[0][k] = (T)(tau * cfg.w[k] * cfg.ih0);
+    s.p[1][k] = (T)(tau * cfg.w[k] * cfg.ih1);
+    s.q[0][k] = (T)(sigma * cfg.w[k] * cfg.ih0);
+    s.q[1][k] = (T)(sigma * cfg.w[k] * cfg.ih1);
+  }
+  return s;
+}
+
+// x+ = prox_{tau G}(x - tau A^T y) at pixel (r, c), channel k, from x, f and
+// y = (y0, y1) there, y0 of the pixel above and y1 of the pixel to the left
+// (the terms adjoint_px masks at the image boundary are dropped by selects
+// when M); L1: the data kind
+template <typename T, int C, bool L1, bool M>
+__device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int c, int H, int W, int k,
+                              const FoldedStep<T, C>& s) {
+  if (M) {
+    if (!(r < H - 1)) y0 = T(0);
+    if (!(r > 0)) yup0 = T(0);
+    if (!(c < W - 1)) y1 = T(0);
+    if (!(c > 0)) yleft1 = T(0);
+  }
+  const T v = x + (y0 - yup0) * s.p[0][k] + (y1 - yleft1) * s.p[1][k];
+  if constexpr (L1) {
+    const T d = v - f;
+    return f + (d > s.b ? d - s.b : (d < -s.b ? d + s.b : T(0)));
+  } else {
+    return s.a * (v + s.b * f);
+  }
+}
+
+// y+ = proj_{|.| <= tvw}(y + sigma A xbar) at pixel (r, c) from xbar there,
+// below and to the right (differences beyond the image are 0 when M)
+template <typename T, int C, bool M>
+__device__ inline void dual_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
+                               const FoldedStep<T, C>& s, T tvw, T* yp) {
+  T n2 = T(0);
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    T gd = xbd[k] - xb[k], gr = xbr[k] - xb[k];
+    if (M && !(r < H - 1)) gd = T(0);
+    if (M && !(c < W - 1)) gr = T(0);
+    yp[k] = y[k] + gd * s.q[0][k];
+    yp[C + k] = y[C + k] + gr * s.q[1][k];
+    n2 += yp[k] * yp[k] + yp[C + k] * yp[C + k];
+  }
+  if constexpr (sizeof(T) == 4) {
+    // min(1, tvw / |v|): 1 inside the ball (and for v = 0, where the
+    // reciprocal square root is +inf)
+    const T sc = __builtin_fminf(T(1), ball_scale(tvw, n2));
+#pragma unroll
+    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+  } else if (n2 > tvw * tvw) {
+    const T sc = ball_scale(tvw, n2);
+#pragma unroll
+    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+  }
+}
+
 // ---- one pass per iteration -------------------------------------------------
 // Cross-lane moves.  DPP wave shifts (one VALU instruction): shr1 gives lane
 // L the value of lane L-1 and lane 0 `edge`; shl1 gives lane L the value of

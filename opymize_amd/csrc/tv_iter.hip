@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
   constexpr int NX = RPT + 1 + OX;    // rows with x+
   constexpr int NY = NX + 1;          // y row j <-> image row rs - OX - 1 + j
   constexpr int NP = RPT + OX;        // rows with y+ (image rows rs - OX ..)
+  constexpr bool kFold = !kAdapt && !PC;  // scalar steps, no residuals: folded stencils
   static_assert(NY < 64, "the transposed halo needs a lane per row");
   __shared__ double sm[2 * (kTB / 64)][4];
   int parity = 0;
@@ -49,6 +50,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
     const T sigma = (T)(st[tr.b].sigma / th_d);
     const PrimalStep<T, C, PC> prox(tau_s, I, cfg);
     const DualStep<T, C, PC> ds(sigma, cfg);
+    const FoldedStep<T, C> fs = fold_step<T, C>(tau_s, st[tr.b].sigma / th_d, th_d, I, cfg);
     const T tvw = (T)I.tvw;
     const int c0 = tr.c0;
     const int c = c0 + lane;
@@ -113,14 +115,28 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
       for (int k = 0; k < C; ++k) yl1[k] = dpp_shr1(readlane(ylh[C + k], j + 1), yv[j + 1][C + k]);
       adjoint_px<T, C>(yv[j + 1], yv[j + 1] + C, yv[j], yl1, rs - OX + j, c, H, W, cfg, out);
     };
+    // scalar steps (constant, accelerated): the folded stencil
+    auto fold_prox = [&](T x, T fx, const T* y, const T* yup, const T* yl1, int r, int cc, int k) {
+      return I.data == OPY_TV_DATA_L1
+                 ? primal_px<T, C, true, true>(x, fx, y[k], y[C + k], yup[k], yl1[k], r, cc, H, W, k, fs)
+                 : primal_px<T, C, false, true>(x, fx, y[k], y[C + k], yup[k], yl1[k], r, cc, H, W, k, fs);
+    };
     T xp[NX][C];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
       const int r = rs - OX + j;
-      T aty[C];
-      aty_row(j, aty);
+      if constexpr (kFold) {
+        T yl1[C];
 #pragma unroll
-      for (int k = 0; k < C; ++k) xp[j][k] = prox(xv[j][k], aty[k], fv[j][k], r, c, H, W, k, cfg);
+        for (int k = 0; k < C; ++k) yl1[k] = dpp_shr1(readlane(ylh[C + k], j + 1), yv[j + 1][C + k]);
+#pragma unroll
+        for (int k = 0; k < C; ++k) xp[j][k] = fold_prox(xv[j][k], fv[j][k], yv[j + 1], yv[j], yl1, r, c, k);
+      } else {
+        T aty[C];
+        aty_row(j, aty);
+#pragma unroll
+        for (int k = 0; k < C; ++k) xp[j][k] = prox(xv[j][k], aty[k], fv[j][k], r, c, H, W, k, cfg);
+      }
     }
     // ---- right halo, transposed: lane k forms x+ at (x row k, cr)
     T xbh[C], xdh[C];
@@ -138,10 +154,11 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         }
         yl1[k] = sel;
       }
-      adjoint_px<T, C>(yo, yo + C, yrh, yl1, rx, cr, H, W, cfg, ah);
+      if (!kFold) adjoint_px<T, C>(yo, yo + C, yrh, yl1, rx, cr, H, W, cfg, ah);
 #pragma unroll
       for (int k = 0; k < C; ++k) {
-        const T p = prox(xrh[k], ah[k], frh[k], rx, cr, H, W, k, cfg);
+        const T p = kFold ? fold_prox(xrh[k], frh[k], yo, yrh, yl1, rx, cr, k)
+                          : prox(xrh[k], ah[k], frh[k], rx, cr, H, W, k, cfg);
         xbh[k] = (T(1) + theta) * p - theta * xrh[k];
         xdh[k] = xrh[k] - p;
       }
@@ -198,7 +215,8 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         xbd[k] = xbar(i + 1, k);
         xbr[k] = dpp_shl1(readlane(xbh[k], i), xbi[k]);
       }
-      yplus_px<T, C>(xbi, xbd, xbr, yv[i + 1], r, c, H, W, ds, tvw, cfg, yp, gb);
+      if constexpr (kFold) dual_px<T, C, true>(xbi, xbd, xbr, yv[i + 1], r, c, H, W, fs, tvw, yp);
+      else yplus_px<T, C>(xbi, xbd, xbr, yv[i + 1], r, c, H, W, ds, tvw, cfg, yp, gb);
       if (mine && ok) {
         const int64_t p = (int64_t)r * W + c;
         store_c<T, C>(xob + p * C, xp[i]);

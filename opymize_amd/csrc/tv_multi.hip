@@ -7,15 +7,21 @@ namespace {
 
 // LDS bytes of one block: y (2C components) and xbar (C) over the region of
 // a tile (the 64 x 4*RPT tile and K pixels on every side, rows rounded up to
-// a multiple of the 4 waves) with a guard pixel on every side.
+// a multiple of the 4 waves) with a guard pixel on every side, and the K
+// iterations' scalar steps.
+constexpr int multi_lds(int C, int rpt, int k, int size) {
+  return (4 * ((4 * rpt + 2 * k + 3) / 4) + 2) * (64 + 2 * k + 2) * 3 * C * size + k * (3 + 4 * C) * size;
+}
 template <typename T, int C, int RPT, int K>
 constexpr int multi_lds_bytes() {
-  return (4 * ((4 * RPT + 2 * K + 3) / 4) + 2) * (64 + 2 * K + 2) * 3 * C * (int)sizeof(T);
+  static_assert(sizeof(FoldedStep<T, C>) == (3 + 4 * C) * sizeof(T), "step table layout");
+  return multi_lds(C, RPT, K, (int)sizeof(T));
 }
 constexpr int kMultiLdsMax = 64 * 1024;
 
 // K iterations x -> x+, y -> y+ of the single-pass kernel (same per-pixel
-// arithmetic: adjoint_px, PrimalStep, xbar, yplus_px) on one tile per block,
+// arithmetic: primal_px, xbar, dual_px with scalar steps; adjoint_px,
+// PrimalStep, yplus_px with preconditioned ones) on one tile per block,
 // with one read of x, f, y and one write of x+, y+ per K iterations
 // (28 B per pixel at C = 1 FP32 for K iterations instead of 28 K B).
 //
@@ -46,9 +52,10 @@ constexpr int kMultiLdsMax = 64 * 1024;
 //
 // Steps: tau, sigma (and theta with accelerated steps) of iteration i are
 // derived from the instance's state exactly as the single-step launches and
-// opy_tv_accel would advance it, by one thread per tile into a small LDS
-// table (this FP64 work was a fifth of the instruction issue when every wave
-// did it); the state itself is not written.
+// opy_tv_accel would advance it, and folded into the stencil weights
+// (fold_step), by one thread per tile into a small LDS table (this FP64 work
+// was a fifth of the instruction issue when every wave did it); the state
+// itself is not written.
 template <typename T, int C, int RPT, int K, bool PC>
 __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ xin, T* __restrict__ xout,
                                                        const T* __restrict__ yin, T* __restrict__ yout,
@@ -69,7 +76,7 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
   static_assert(multi_lds_bytes<T, C, RPT, K>() <= kMultiLdsMax, "LDS budget");
   __shared__ T sy[2 * C][NP];
   __shared__ T sxb[C][NP];
-  __shared__ T sstep[K][5];  // scalar steps per iteration: tau, a, b (prox coefficients), theta, dual step
+  __shared__ FoldedStep<T, C> sstep[K];  // scalar steps of the K iterations
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
@@ -133,12 +140,7 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
       double tau_d = st[tr.b].tau, sig_d = st[tr.b].sigma;
       for (int i = 0; i < K; ++i) {
         const double th_d = accel_theta(cfg.steps, I, tau_d);
-        const PrimalStep<T, C, false> p(tau_d, I, cfg);
-        sstep[i][0] = p.tau[0];
-        sstep[i][1] = p.a[0];
-        sstep[i][2] = p.b[0];
-        sstep[i][3] = (T)th_d;
-        sstep[i][4] = (T)(sig_d / th_d);
+        sstep[i] = fold_step<T, C>(tau_d, sig_d / th_d, th_d, I, cfg);
         tau_d *= th_d;
         sig_d /= th_d;
       }
@@ -150,10 +152,8 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
     const DualStep<T, C, PC> ds_pc(T(0), cfg);
 #pragma unroll 1
     for (int i = 1; i <= K; ++i) {
-      const T* sp = sstep[i - 1];
-      const T theta = PC ? T(1) : sp[3];
-      const PrimalStep<T, C, PC> prox = PC ? prox_pc : PrimalStep<T, C, PC>(sp[0], sp[1], sp[2], I);
-      const DualStep<T, C, PC> ds = PC ? ds_pc : DualStep<T, C, PC>(sp[4], cfg);
+      const FoldedStep<T, C> fs = sstep[i - 1];  // unused with preconditioned steps
+      const T theta = PC ? T(1) : fs.theta;
       // one iteration; L1: data kind, M: boundary masks (both uniform over the tile)
       auto iterate = [&](auto l1_tag, auto m_tag) {
         constexpr bool L1 = decltype(l1_tag)::value, M = decltype(m_tag)::value;
@@ -169,10 +169,12 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
             yup0[k] = sy[k][q - PW];
             yl1[k] = sy[C + k][q - 1];
           }
-          adjoint_px<T, C, M>(yv[j], yv[j] + C, yup0, yl1, r, c, H, W, cfg, aty);
+          if constexpr (PC) adjoint_px<T, C, M>(yv[j], yv[j] + C, yup0, yl1, r, c, H, W, cfg, aty);
 #pragma unroll
           for (int k = 0; k < C; ++k) {
-            const T p = prox.template apply<L1, M>(xv[j][k], aty[k], fv[j][k], r, c, H, W, k, cfg);
+            const T p = PC ? prox_pc.template apply<L1, M>(xv[j][k], aty[k], fv[j][k], r, c, H, W, k, cfg)
+                           : primal_px<T, C, L1, M>(xv[j][k], fv[j][k], yv[j][k], yv[j][C + k], yup0[k], yl1[k], r,
+                                                    c, H, W, k, fs);
             sxb[k][q] = (T(1) + theta) * p - theta * xv[j][k];
             xv[j][k] = p;
           }
@@ -191,7 +193,8 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
             xbd[k] = sxb[k][q + PW];
             xbr[k] = sxb[k][q + 1];
           }
-          yplus_px<T, C, PC, M>(xbc, xbd, xbr, yv[j], r, c, H, W, ds, tvw, cfg, yp, gb);
+          if constexpr (PC) yplus_px<T, C, PC, M>(xbc, xbd, xbr, yv[j], r, c, H, W, ds_pc, tvw, cfg, yp, gb);
+          else dual_px<T, C, M>(xbc, xbd, xbr, yv[j], r, c, H, W, fs, tvw, yp);
 #pragma unroll
           for (int e = 0; e < 2 * C; ++e) {
             yv[j][e] = yp[e];
@@ -236,11 +239,6 @@ void launch_multi(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_
 }
 
 template <typename T>
-int multi_lds(int C, int rpt, int k) {
-  return (4 * ((4 * rpt + 2 * k + 3) / 4) + 2) * (64 + 2 * k + 2) * 3 * C * (int)sizeof(T);
-}
-
-template <typename T>
 int multi_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_inst_t* insts,
                const opy_adapt_state_t* st, const int32_t* ai, const int32_t* ao, const int32_t* na,
                const opy_tv_cfg_t* cfg, int k, int grid, void* s) {
@@ -249,9 +247,9 @@ int multi_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_
   if (cfg->rpt != 2 && cfg->rpt != 4 && cfg->rpt != 6 && cfg->rpt != 8)
     return fail(kInvalid, "tv_multi: rows per thread %d not in {2,4,6,8}", cfg->rpt);
   if (k != 2 && k != 3 && k != 4) return fail(kInvalid, "tv_multi: k = %d not in {2,3,4}", k);
-  if (multi_lds<T>(cfg->C, cfg->rpt, k) > kMultiLdsMax)
+  if (multi_lds(cfg->C, cfg->rpt, k, (int)sizeof(T)) > kMultiLdsMax)
     return fail(kInvalid, "tv_multi: C=%d, rpt=%d, k=%d needs %d B of LDS (at most %d)", cfg->C, cfg->rpt, k,
-                multi_lds<T>(cfg->C, cfg->rpt, k), kMultiLdsMax);
+                multi_lds(cfg->C, cfg->rpt, k, (int)sizeof(T)), kMultiLdsMax);
   if (!xi || !xo || !yi || !yo || !f || !insts || !st || !ai || !ao || !na || grid < 0)
     return fail(kInvalid, "tv_multi: null pointer or negative grid");
   if (xi == xo || yi == yo) return fail(kInvalid, "tv_multi: ping-pong buffers must differ");
