@@ -367,6 +367,22 @@ int opy_tv_accel(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy
 /* the same for `count` consecutive iterations in one launch (after opy_tv_multi_* with k = count) */
 int opy_tv_accel_n(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks, int B,
                    int count, void* stream);
+/* Restarts of the accelerated schedule (no reference counterpart; O'Donoghue & Candes 2015 restart
+ * accelerated methods, Applegate et al. 2021 restart PDHG).  After opy_tv_check: every running
+ * TV-L2 instance whose certified relative gap g = |relgap_cert| has fallen below beta times its
+ * value at the last restart, or has risen above its smallest value since that restart, with at
+ * least min_iters iterations run since, returns to the initial steps tau0, sigma0 (x and y stay).
+ * One record per instance; initialise ref = best = +inf, last = 0 (the iteration count of the
+ * start). */
+typedef struct {
+  double ref;   /* g at the last restart (+inf: none recorded yet) */
+  double best;  /* smallest g since the last restart (+inf: none yet) */
+  int32_t last; /* iterations run at the last restart */
+  int32_t count;/* restarts so far */
+} opy_tv_restart_t;
+int opy_tv_restart(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks,
+                   opy_tv_restart_t* records, int B, double beta, int min_iters, double tau0, double sigma0,
+                   void* stream);
 /* FP64 gap partials per tile (reference pdhg.py:23-44) */
 int opy_tv_gap_f32(const float* x, const float* y, const float* f, const opy_tv_inst_t* insts,
                    const int32_t* active_inst, const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg,

@@ -69,6 +69,37 @@ __global__ void __launch_bounds__(kTB) tv_accel_kernel(const opy_tv_inst_t* __re
   st[b].sigma = sigma;
 }
 
+// ---- restarts of the accelerated schedule ------------------------------------
+// After a check, per instance (one thread each): a running TV-L2 instance
+// restarts its step schedule at tau0, sigma0 when g = |relgap_cert| has
+// fallen below beta times g at the last restart (sufficient decay) or has
+// risen above the smallest g since then (no progress), once min_iters
+// iterations have run since the last restart.  The decay test needs a
+// recorded g; the first check only records it.
+__global__ void __launch_bounds__(kTB) tv_restart_kernel(const opy_tv_inst_t* __restrict__ insts,
+                                                         opy_adapt_state_t* __restrict__ st,
+                                                         const opy_tv_check_t* __restrict__ chk,
+                                                         opy_tv_restart_t* __restrict__ rec, int B, double beta,
+                                                         int min_iters, double tau0, double sigma0) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B || chk[b].status != 0 || insts[b].data != OPY_TV_DATA_L2) return;
+  const double g = fabs(chk[b].relgap_cert);
+  if (!isfinite(g)) return;
+  opy_tv_restart_t R = rec[b];
+  const bool fire = chk[b].iters - R.last >= min_iters && ((isfinite(R.ref) && g < beta * R.ref) || g > R.best);
+  R.best = fmin(R.best, g);
+  if (!isfinite(R.ref)) R.ref = g;
+  if (fire) {
+    st[b].tau = tau0;
+    st[b].sigma = sigma0;
+    R.ref = g;
+    R.best = INFINITY;
+    R.last = chk[b].iters;
+    R.count += 1;
+  }
+  rec[b] = R;
+}
+
 // ---- primal-dual gap (reference pdhg.py:23-44) ------------------------------
 // per tile partials: [P_data, P_tv, D_lin = sum z f, D_quad = sum z^2, max |y_p|, max |z|] with z = A^T y
 template <typename T, int C, int RPT>
@@ -310,6 +341,16 @@ OPY_API int opy_tv_accel_n(const opy_tv_inst_t* insts, opy_adapt_state_t* st, co
   hipLaunchKernelGGL(tv_accel_kernel, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, as_stream(s), insts, st, chk, B,
                      count);
   return check_launch("tv_accel_n");
+}
+OPY_API int opy_tv_restart(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk,
+                           opy_tv_restart_t* rec, int B, double beta, int min_iters, double tau0, double sigma0,
+                           void* s) {
+  if (!insts || !st || !chk || !rec || B < 1 || !(beta >= 0.0) || min_iters < 0 || !(tau0 > 0.0) ||
+      !(sigma0 > 0.0))
+    return fail(kInvalid, "tv_restart: bad arguments");
+  hipLaunchKernelGGL(tv_restart_kernel, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, as_stream(s), insts, st, chk, rec, B,
+                     beta, min_iters, tau0, sigma0);
+  return check_launch("tv_restart");
 }
 OPY_API int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* st, const opy_tv_check_t* chk,
                          const double* partials, int B, int C, const opy_adapt_params_t* p, void* s) {

@@ -49,9 +49,20 @@ class TVCheck(ctypes.Structure):
                [("status", ctypes.c_int32), ("iters", ctypes.c_int32)]
 
 
+class TVRestart(ctypes.Structure):
+    _fields_ = [("ref", ctypes.c_double), ("best", ctypes.c_double), ("last", ctypes.c_int32),
+                ("count", ctypes.c_int32)]
+
+
+# restarts of the accelerated schedule: sufficient decay factor of the
+# certified relative gap, and the fewest iterations between two restarts
+RESTART_DEFAULT = (0.3, 96)
+
 CHECK_DOUBLES = ctypes.sizeof(TVCheck) // 8
+RESTART_DOUBLES = ctypes.sizeof(TVRestart) // 8
 STATE_DOUBLES = _hip.ADAPT_STATE_DOUBLES
 assert ctypes.sizeof(TVInst) == 48 and ctypes.sizeof(TVCfg) == 64 and ctypes.sizeof(TVCheck) == 72
+assert ctypes.sizeof(TVRestart) == 24
 
 _SIGS = {
     "opy_tv_primal": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p],
@@ -65,6 +76,8 @@ _PLAIN = {
     "opy_tv_adapt": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
     "opy_tv_accel": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_accel_n": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_restart": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_void_p],
     "opy_tv_check": [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_compact": [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int,
@@ -158,6 +171,7 @@ class TVBatch(object):
         self.set_tile(*tile)
         self.graph = None
         self.cur = 0  # buffer holding the current iterate
+        self._restart = None  # (beta, min_iters, tau0, sigma0) while a solve restarts its schedule
 
     # -- layout --------------------------------------------------------------
     def set_multi(self, k):
@@ -277,6 +291,9 @@ class TVBatch(object):
         if statuses is not None:
             chk.view(np.int32)[:, 2 * 8] = statuses
         self.checks = torch.as_tensor(chk).cuda()
+        rec = np.zeros((self.B, RESTART_DOUBLES))
+        rec[:, :2] = np.inf  # ref, best: none yet; last = count = 0
+        self.restarts = torch.as_tensor(rec).cuda()
         self.iter_counter.zero_()
         self._compact(0)
 
@@ -349,6 +366,11 @@ class TVBatch(object):
 
     def _check_step(self, cfg, tol, tol_infeas, certify, iters_since):
         self._gap_check(cfg, tol, tol_infeas, certify, iters_since)
+        if self._restart is not None:
+            beta, min_iters, tau0, sigma0 = self._restart
+            self._call("opy_tv_restart", _hip.ptr(self.insts), _hip.ptr(self.state), _hip.ptr(self.checks),
+                       _hip.ptr(self.restarts), self.B, float(beta), int(min_iters), float(tau0), float(sigma0),
+                       _hip.stream())
         self._compact(iters_since)
 
     def _graph_block(self, cache, fn):
@@ -416,18 +438,32 @@ class TVBatch(object):
                 st[b, F["sigma"]] = np.sqrt(step_bound / step_factor)
                 st[b, F["tau"]] = step_bound / st[b, F["sigma"]]
         self.state.copy_(torch.as_tensor(st))
+        rec = self.restarts.cpu().numpy()
+        iters = self.checks.cpu().numpy().view(np.int32)[:, 17]
+        for b in np.flatnonzero(flags):
+            rec[b, :2] = np.inf
+            rec[b:b + 1, 2:3].view(np.int32)[0, 0] = iters[b]
+        self.restarts.copy_(torch.as_tensor(rec))
         self._compact(0)
 
     def solve(self, steps="adaptive", step_bound=0.124, step_factor=1.0, tol=1e-5, tol_infeas=None,
               max_iters=20000, block_iters=50, check_every=1, certify=True, use_graph=True, eps_res=None,
-              callback=None, check_start=False, keep_state=False, fallback=False):
+              callback=None, check_start=False, keep_state=False, fallback=False, restart="auto"):
         """Iterate until every instance is certified (|relgap| < tol and
         infeasibility < tol_infeas) or max_iters; the test runs every
         block_iters*check_every iterations.  Returns self.results().
 
         fallback=True (for warm starts): the gap of the start point is
         recorded, and an instance whose gap has not improved at the first
-        test (or is not finite) restarts cold from x = y = 0."""
+        test (or is not finite) restarts cold from x = y = 0.
+
+        restart (accelerated steps): "auto" or True restarts the step
+        schedule of a TV-L2 instance at its initial tau, sigma (keeping x,
+        y) when, at a check at least min_iters iterations after its last
+        restart, its certified relative gap has fallen below beta times the
+        gap at that restart or has risen above the smallest gap since;
+        (beta, min_iters) sets the two, RESTART_DEFAULT otherwise; False
+        or None keeps the plain schedule."""
         if steps not in STEP_MODES:
             raise ValueError("steps must be one of %s" % sorted(STEP_MODES))
         tol_infeas = tol if tol_infeas is None else tol_infeas
@@ -443,6 +479,11 @@ class TVBatch(object):
         params.eps_absp = params.eps_relp = params.eps_absd = params.eps_reld = e
         if not keep_state:
             self._reset_state(steps, step_bound, step_factor, tol, tol_infeas, certify)
+        self._restart = None
+        if steps == "accel" and restart not in (False, None):
+            beta, min_iters = RESTART_DEFAULT if restart in ("auto", True) else restart
+            sigma0 = np.sqrt(step_bound / step_factor)
+            self._restart = (float(beta), int(min_iters), step_bound / sigma0, sigma0)
         self.cur = 0
         if check_start:
             self._check_step(cfg, tol, tol_infeas, certify, 0)
@@ -499,13 +540,15 @@ class TVBatch(object):
 
     def results(self):
         chk = self.checks.cpu().numpy()
+        rv = self.restarts.cpu().numpy().view(np.int32)
         out = []
         iv = chk.view(np.int32)
         for b in range(self.B):
             c = chk[b]
             out.append({"objp": c[0], "objd": c[1], "gap": c[2], "relgap": c[3], "infeasp": c[4],
                         "infeasd": c[5], "objd_cert": c[6], "relgap_cert": c[7],
-                        "status": int(iv[b, 16]), "iter": int(iv[b, 17])})
+                        "status": int(iv[b, 16]), "iter": int(iv[b, 17]),
+                        "restarts": int(rv[b, 5])})
         return out
 
     def steps_state(self):

@@ -95,9 +95,17 @@ def pd_gap(prob, x, Ax, ATy, y):
 
 
 def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-5, term_maxiter=1000,
-          granularity=5000, x0=None, y0=None, term_pd_res=None, history=False):
+          granularity=5000, x0=None, y0=None, term_pd_res=None, history=False, restart=None):
     """Iterate exactly as the reference's CPU path; returns (x, y, info, hist).
-    hist (if requested) holds per-iteration (tau, sigma, alpha, resp, resd)."""
+    hist (if requested) holds per-iteration (tau, sigma, alpha, resp, resd).
+
+    restart = (beta, min_iters), accelerated steps on a strongly convex G
+    only (no reference counterpart): at a test, at least min_iters
+    iterations after the last restart, where |relgap| has fallen below beta
+    times its value at that restart or has risen above its smallest value
+    since, tau and sigma return to their initial values (x, y stay); the
+    first test only records the gap.  info["restarts"] lists the iterations
+    after which they happened."""
     A, AT = prob.A, prob.AT
     m, n = A.shape
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
@@ -132,6 +140,7 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
             tau = bnd / sigma
     hist = []
     it = 0
+    rs_ref, rs_best, rs_last, restarts = np.inf, np.inf, 0, []
     while it < term_maxiter:
         it += 1
         xn = x - tau * xgrad
@@ -175,7 +184,21 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
                     term_pd_gap[1]
             if done:
                 break
+            if steps == "accel" and restart is not None and prob.gamma > 0 and not pd_res_mode:
+                g = abs(info["relgap"])
+                if np.isfinite(g):
+                    fire = it - rs_last >= restart[1] and ((np.isfinite(rs_ref) and g < restart[0] * rs_ref)
+                                                          or g > rs_best)
+                    rs_best = min(rs_best, g)
+                    if not np.isfinite(rs_ref):
+                        rs_ref = g
+                    if fire:
+                        sigma = np.sqrt(bnd / step_factor)
+                        tau = bnd / sigma
+                        rs_ref, rs_best, rs_last = g, np.inf, it
+                        restarts.append(it)
     info["iter"] = it
+    info["restarts"] = restarts
     if steps == "adaptive":
         info.update(tau=tau, sigma=sigma, alpha=alpha)
     elif steps == "accel":

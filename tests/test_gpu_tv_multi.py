@@ -17,9 +17,10 @@ pytestmark = pytest.mark.gpu
 def lds_bytes(precision, C, rpt, k):
     """LDS of one block: y and xbar (3C values) over the tile and k pixels
     around it, rows rounded up to a multiple of 4, and a guard pixel on every
-    side."""
+    side; and the k iterations' scalar steps (3 + 4C values each)."""
     rows = 4 * ((4 * rpt + 2 * k + 3) // 4) + 2
-    return rows * (64 + 2 * k + 2) * 3 * C * (8 if precision == "double" else 4)
+    size = 8 if precision == "double" else 4
+    return rows * (64 + 2 * k + 2) * 3 * C * size + k * (3 + 4 * C) * size
 
 
 def _batch(precision, tile, multi, shapes=SHAPES, C=1, seed=0, data=None):
@@ -141,6 +142,47 @@ def test_multi_converges_and_certifies(gpu):
         # 0.5 lam |x - x*|^2 <= gap for either point
         bound = np.sqrt(2 * abs(res[b]["gap"]) / lams[b]) + np.sqrt(2 * abs(io["gap"]) / lams[b])
         assert np.linalg.norm(xs[b] - xo) <= bound + 1e-4, b
+
+
+@pytest.mark.parametrize("multi", [1, 4])
+def test_restarts_match_oracle(gpu, multi):
+    """Restarts of the accelerated schedule (FP64, tests every 20
+    iterations, beta 0.3, at least 40 iterations apart): the same restarts
+    and iterates as the oracle's rule on the uneven batch; TV-L1 instances
+    (no acceleration) never restart."""
+    bat, imgs, lams, data = _batch("double", (64, 2), multi)
+    k = 300
+    res = bat.solve(steps="accel", step_bound=0.124, step_factor=4.0, max_iters=k, block_iters=20, tol=1e-30,
+                    restart=(0.3, 40))
+    xs, ys = bat.solution()
+    total = 0
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps="accel", step_bound=0.124, step_factor=4.0, term_maxiter=k,
+                                  granularity=20, term_pd_gap=1e-30, restart=(0.3, 40))
+        assert res[b]["restarts"] == len(io["restarts"]), (b, res[b]["restarts"], io["restarts"])
+        if data[b] == "l1":
+            assert res[b]["restarts"] == 0
+        total += res[b]["restarts"]
+        assert np.abs(xs[b] - xo).max() < 1e-9, b
+        assert np.abs(ys[b] - yo).max() < 1e-9, b
+    assert total > 0
+
+
+def test_restarts_certify_a_stalling_instance(gpu):
+    """FP32, accelerated steps: the instance with lambda = 1.29 of this batch
+    does not reach a certified relative gap of 1e-5 in 20000 iterations
+    without restarts (its gap stops falling); with the default restarts
+    every instance certifies, in fewer iterations in total."""
+    out = {}
+    for r in (False, "auto"):
+        bat, imgs, lams, data = _batch("float", (64, 8), 4, seed=3, data="l2")
+        out[r] = bat.solve(steps="accel", step_bound=0.124, max_iters=20000, block_iters=40, tol=1e-5, restart=r)
+    assert out[False][0]["status"] == 0
+    assert all(r["status"] == 1 for r in out["auto"])
+    assert sum(r["iter"] for r in out["auto"]) < sum(r["iter"] for r in out[False])
+    assert max(r["restarts"] for r in out["auto"]) > 0 and max(r["restarts"] for r in out[False]) == 0
 
 
 def test_multi_rejects_bad_arguments(gpu):

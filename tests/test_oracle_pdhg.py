@@ -122,3 +122,69 @@ def test_accelerated_steps_rate_and_reduction():
     xc, yc, _, _ = pdhg.solve(prob, steps="const", step_bound=0.124, term_maxiter=50, granularity=10 ** 9)
     np.testing.assert_allclose(xa, xc, rtol=0, atol=1e-14)
     np.testing.assert_allclose(ya, yc, rtol=0, atol=1e-14)
+
+
+def test_restart_equals_fresh_solve_from_the_iterate():
+    """A restart of the accelerated schedule keeps (x, y) and returns tau,
+    sigma to their initial values: the restarted run equals a plain run up
+    to the restart followed by a fresh plain run started from that iterate,
+    and restarts fall on tests at least min_iters apart."""
+    H, W, lam = 12, 14, 1.5
+    f = _rects(H, W, 5).reshape(-1, 1)
+    prob = pdhg.tv_problem(f, (H, W), lam)
+    kw = dict(steps="accel", step_bound=0.124, step_factor=4.0, term_pd_gap=1e-30, granularity=10)
+    x, y, info, _ = pdhg.solve(prob, term_maxiter=400, restart=(0.3, 20), **kw)
+    rs = info["restarts"]
+    assert len(rs) >= 2, rs
+    assert all(r % 10 == 0 for r in rs) and rs[0] >= 20 and np.all(np.diff(rs) >= 20), rs
+    k1, k2 = rs[0], rs[1]
+    x1, y1, i1, _ = pdhg.solve(prob, term_maxiter=k1, **kw)
+    x2, y2, i2, _ = pdhg.solve(prob, term_maxiter=k2 - k1, x0=x1, y0=y1, **kw)
+    xr, yr, ir, _ = pdhg.solve(prob, term_maxiter=k2, restart=(0.3, 20), **kw)
+    assert ir["restarts"] == [k1, k2]
+    np.testing.assert_array_equal(xr, x2)
+    np.testing.assert_array_equal(yr, y2)
+    # the plain run diverges from the restarted one after the first restart
+    xp, _, _, _ = pdhg.solve(prob, term_maxiter=k2, **kw)
+    assert np.abs(xp - xr).max() > 1e-8
+
+
+def test_restart_rule_on_a_gap_sequence():
+    """The firing rule against a hand-worked sequence of tests: with
+    min_iters = 20 and tests every 10 iterations, the first test records the
+    gap only, a rise above the smallest gap since the last restart or a fall
+    below beta times the gap at it fires, and not within 20 iterations of
+    the last restart."""
+    def rule(gaps, beta=0.3, min_iters=20, every=10):
+        ref, best, last, out = np.inf, np.inf, 0, []
+        for i, g in enumerate(gaps):
+            it = (i + 1) * every
+            fire = it - last >= min_iters and ((np.isfinite(ref) and g < beta * ref) or g > best)
+            best = min(best, g)
+            if not np.isfinite(ref):
+                ref = g
+            if fire:
+                ref, best, last = g, np.inf, it
+                out.append(it)
+        return out
+    # 10: record ref 1.0; 20: 0.5 no (0.5 >= 0.3, no rise); 30: 0.6 > 0.5 rise -> fire;
+    # 40: within 20 of 30; 50: 0.1 < 0.3 * 0.6 -> fire; 60, 70: falling slowly; 80: rise -> fire
+    gaps = [1.0, 0.5, 0.6, 0.55, 0.1, 0.09, 0.085, 0.2]
+    assert rule(gaps) == [30, 50, 80]
+    # the oracle's solve applies the same rule: replay its own test values
+    H, W, lam = 10, 9, 1.2
+    prob = pdhg.tv_problem(_rects(H, W, 2).reshape(-1, 1), (H, W), lam)
+    seen = []
+    orig = pdhg.pd_gap
+
+    def spy(*a):
+        r = orig(*a)
+        seen.append(abs(r["relgap"]))
+        return r
+    pdhg.pd_gap = spy
+    try:
+        _, _, info, _ = pdhg.solve(prob, steps="accel", step_bound=0.124, step_factor=4.0, term_pd_gap=1e-30,
+                                   term_maxiter=300, granularity=10, restart=(0.3, 20))
+    finally:
+        pdhg.pd_gap = orig
+    assert info["restarts"] and info["restarts"] == rule(seen)
