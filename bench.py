@@ -9,7 +9,9 @@ solver from that start until every instance is certified or the iteration
 cap, with a cold restart for any instance whose gap does not improve at the
 first check (--cold: from x = y = 0 instead).  The batched solver runs the
 accelerated steps of Chambolle & Pock (2011, Algorithm 2) on these strongly
-convex instances (--step-rule adaptive: the reference's rule).  After the
+convex instances, restarting an instance's step schedule when its certified
+gap has fallen enough or stopped falling (--no-restart: never; --step-rule
+adaptive: the reference's rule).  After the
 timed steps the other start (cold, or warm with --cold) is solved once more
 for comparison.
 
@@ -220,6 +222,8 @@ def main():
     ap.add_argument("--step-factor", type=float, default=4.0,
                     help="initial tau/sigma split: tau = sqrt(bound * factor), sigma = sqrt(bound / factor)")
     ap.add_argument("--no-compare", action="store_true", help="skip the solve from the other start")
+    ap.add_argument("--no-restart", action="store_true",
+                    help="accelerated steps without restarts of the step schedule")
     ap.add_argument("--step-rule", default="accel", choices=["accel", "adaptive"],
                     help="step rule of the batched solver")
     ap.add_argument("--plain", type=int, default=6, help="instances for the plain single-instance baseline")
@@ -256,7 +260,7 @@ def main():
             bat.set_start(None, None)
         return bat.solve(steps=args.step_rule, step_bound=0.124, step_factor=args.step_factor,
                          max_iters=args.max_iters, block_iters=args.block_iters, tol=1e-5, certify=True,
-                         fallback=warm)
+                         fallback=warm, restart=not args.no_restart)
 
     warm = not args.cold
 
@@ -267,11 +271,13 @@ def main():
     certified = 0
     iters = []
     fallbacks = 0
+    nrestarts = []
     for _ in range(args.steps):
         res = one_step(warm)
         fallbacks += len(bat.fallbacks)
         certified += sum(r["status"] == 1 for r in res)
         iters += [r["iter"] for r in res]
+        nrestarts += [r["restarts"] for r in res]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = certified / elapsed
@@ -319,6 +325,7 @@ def main():
         cpu = cpu_baseline(bat, specs, args.cpu_budget, int(iters[0]), args.step_rule)
         log("cpu oracle: %s" % cpu["sample"])
 
+    from opymize_amd.solvers import tv as tvmod
     line = {
         "metric": "solves/sec to rel PD-gap 1e-5: batched+warm-start vs plain PDHG; %HBM peak",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -333,6 +340,9 @@ def main():
                    "warm_start": ("cnn (6-layer 3x3 conv, 32 channels, bf16 MFMA), cold restart where the gap "
                                   "does not improve (%d instances)" % fallbacks) if warm else "none (cold start)",
                    "step_factor": args.step_factor,
+                   "restarts": None if (args.no_restart or args.step_rule != "accel") else
+                   {"beta": tvmod.RESTART_DEFAULT[0], "min_iters": tvmod.RESTART_DEFAULT[1],
+                    "median_per_instance": float(np.median(nrestarts))},
                    "other_start": other,
                    "tile": list(args.tile), "block_iters": args.block_iters, "iterations_per_launch": k_launch,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
