@@ -56,7 +56,7 @@ class TVRestart(ctypes.Structure):
 
 # restarts of the accelerated schedule: sufficient decay factor of the
 # certified relative gap, and the fewest iterations between two restarts
-RESTART_DEFAULT = (0.3, 96)
+RESTART_DEFAULT = (0.5, 96)
 
 CHECK_DOUBLES = ctypes.sizeof(TVCheck) // 8
 RESTART_DOUBLES = ctypes.sizeof(TVRestart) // 8
