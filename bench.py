@@ -214,7 +214,7 @@ def main():
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--max-iters", type=int, default=10000)
     ap.add_argument("--block-iters", type=int, default=48)
-    ap.add_argument("--tile", type=int, nargs=2, default=[64, 8])
+    ap.add_argument("--tile", type=int, nargs=2, default=[64, 6])
     ap.add_argument("--multi", type=int, default=4, choices=[1, 2, 3, 4],
                     help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
