@@ -417,6 +417,16 @@ int opy_conv3x3_fwd_bf16(const float* x, float* y, const void* wt, const float* 
 int opy_conv3x3_fwd_io(const void* x, void* y, const void* wt, const float* bias, int N, int H, int W, int act,
                        int x_bf16, int y_bf16, int grid, void* stream);
 /* per-block partials (grid x (9*32*32 + 32) floats) of dW[tap][co][ci] and dbias[co] */
+/* The warm start's first layer straight from packed images (no counterpart): input channel 0 =
+ * f - 0.5, channel 1 = loglam[n], channels 2..31 = 0 for image n at f + n*stride (H*W floats);
+ * output NHWC BF16, 32 channels, ReLU. */
+int opy_conv3x3_first(const float* f, int64_t stride, const float* loglam, void* y, const void* wt,
+                      const float* bias, int N, int H, int W, int grid, void* stream);
+/* The last layer into the packed solver start (no counterpart): with o = conv(x) + b (BF16 NHWC
+ * input, no activation), x_out = ((f - 0.5) + 0.5) + o0 and y_out = (o1, o2) per pixel, image n at
+ * f / x_out + n*stride and y_out + 2n*stride. */
+int opy_conv3x3_last(const void* x, const float* f, int64_t stride, float* x_out, float* y_out, const void* wt,
+                     const float* bias, int N, int H, int W, int grid, void* stream);
 int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,
                            void* stream);
 /* out[i] = sum_b partials[b*len + i] in block order */

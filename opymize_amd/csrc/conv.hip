@@ -29,16 +29,33 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline __bf16 to_bf16(float v) { return (__bf16)v; }
 
+// The warm start's first and last layers read and write the solver's packed
+// buffers directly (image n of a group at n * stride elements):
+//   first (kFirst): input channel 0 = f - 0.5, channel 1 = loglam[n], the
+//   other 30 zero, i.e. the network's input encoding, formed while staging;
+//   last (kLast): x = ((f - 0.5) + 0.5) + o0 and y = (o1, o2) per pixel, the
+//   decoding of the output, with only the first 16 output channels computed.
+enum { kPlain = 0, kFirst = 1, kLast = 2 };
+struct ConvIO {
+  const float* f;       // packed images
+  int64_t stride;       // elements between consecutive images of the group
+  const float* loglam;  // kFirst: per image
+  float* xo;            // kLast: image n at xo + n * stride
+  float* yo;            // kLast: image n at yo + 2 n * stride (pairs)
+};
+
 // ---- forward / backward-data -------------------------------------------------
 // y[n,h,w,co] = act(b[co] + sum_{tap,ci} W[tap][co][ci] * x[n, h+dy-1, w+dx-1, ci])
 // act: 0 none, 1 ReLU.  bias may be NULL.  TX / TY: element type of x / y in
 // HBM (float, or __bf16 for activations between layers: the operands are
 // rounded to BF16 for the matrix cores anyway, so a BF16 copy of a layer's
 // output gives the next layer the same operands at half the traffic).
-template <typename TX, typename TY>
+// MODE: kPlain, or kFirst / kLast (x resp. y unused, see ConvIO).
+template <typename TX, typename TY, int MODE = kPlain>
 __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__ x, TY* __restrict__ y,
                                                           const __bf16* __restrict__ wt, const float* __restrict__ bias,
-                                                          int N, int H, int W, int act) {
+                                                          int N, int H, int W, int act, ConvIO io = {}) {
+  constexpr int NJ = MODE == kLast ? 1 : 2;  // blocks of 16 output channels
   __shared__ __align__(16) __bf16 xs[kHalo * kHalo * kPixStride];
   __shared__ __align__(16) __bf16 ws[9 * kCh * kPixStride];
   const int tid = threadIdx.x;
@@ -55,8 +72,21 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
     const int rem = t - n * tx * ty;
     const int h0 = (rem / tx) * kTile, w0 = (rem % tx) * kTile;
     __syncthreads();  // previous tile done with xs (and ws staged)
-    const TX* xb = x + (int64_t)n * H * W * kCh;
-    if constexpr (sizeof(TX) == 4) {
+    if constexpr (MODE == kFirst) {
+      const float* fb = io.f + (int64_t)n * io.stride;
+      const __bf16 ll = to_bf16(io.loglam[n]);
+      for (int e = tid; e < kHalo * kHalo * (kCh / 8); e += 256) {
+        const int q = e % (kCh / 8), p = e / (kCh / 8);
+        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+        bf16x8 v = {};
+        if (q == 0 && hh >= 0 && hh < H && ww >= 0 && ww < W) {
+          v[0] = to_bf16(fb[(int64_t)hh * W + ww] - 0.5f);
+          v[1] = ll;
+        }
+        *reinterpret_cast<bf16x8*>(xs + p * kPixStride + 8 * q) = v;
+      }
+    } else if constexpr (sizeof(TX) == 4) {
+      const TX* xb = x + (int64_t)n * H * W * kCh;
       for (int e = tid; e < kHalo * kHalo * (kCh / 4); e += 256) {
         const int q = e % (kCh / 4), p = e / (kCh / 4);
         const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
@@ -71,6 +101,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
       }
     } else {
       // 8 BF16 channels (16 B) per access
+      const TX* xb = x + (int64_t)n * H * W * kCh;
       for (int e = tid; e < kHalo * kHalo * (kCh / 8); e += 256) {
         const int q = e % (kCh / 8), p = e / (kCh / 8);
         const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
@@ -91,7 +122,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
       const int dy = tap / 3, dx = tap % 3;
       bf16x8 b[2];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < NJ; ++j)
         b[j] = *reinterpret_cast<const bf16x8*>(ws + (tap * kCh + j * 16 + li) * kPixStride + 8 * lk);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
@@ -99,8 +130,31 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
         const bf16x8 a =
             *reinterpret_cast<const bf16x8*>(xs + ((row + dy) * kHalo + li + dx) * kPixStride + 8 * lk);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
+        for (int j = 0; j < NJ; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
       }
+    }
+    if constexpr (MODE == kLast) {
+      // decode: output channels 0..2 of the first block (lanes li < 3)
+      const float* fb = io.f + (int64_t)n * io.stride;
+      float* xo = io.xo + (int64_t)n * io.stride;
+      float* yo = io.yo + 2 * (int64_t)n * io.stride;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int h = h0 + wave * 4 + m;
+        if (h >= H || li >= 3) continue;
+        const float bv = bias ? bias[li] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int w = w0 + 4 * lk + r;
+          if (w >= W) continue;
+          float v = acc[m][0][r] + bv;
+          if (act == 1) v = v > 0.f ? v : 0.f;
+          const int64_t p = (int64_t)h * W + w;
+          if (li == 0) xo[p] = ((fb[p] - 0.5f) + 0.5f) + v;
+          else yo[2 * p + li - 1] = v;
+        }
+      }
+      continue;
     }
     TY* yb = y + (int64_t)n * H * W * kCh;
 #pragma unroll
@@ -337,6 +391,26 @@ OPY_API int opy_conv3x3_fwd_io(const void* x, void* y, const void* wt, const flo
     hipLaunchKernelGGL((conv3x3_fwd_kernel<float, float>), dim3(grid), dim3(256), 0, st,
                        static_cast<const float*>(x), static_cast<float*>(y), w, bias, N, H, W, act);
   return check_launch("conv3x3_fwd_io");
+}
+
+OPY_API int opy_conv3x3_first(const float* f, int64_t stride, const float* loglam, void* y, const void* wt,
+                              const float* bias, int N, int H, int W, int grid, void* s) {
+  if (!f || !loglam || !y || !wt || N < 1 || H < 1 || W < 1 || grid < 1 || stride < (int64_t)H * W)
+    return fail(kInvalid, "conv3x3_first: bad arguments");
+  ConvIO io{f, stride, loglam, nullptr, nullptr};
+  hipLaunchKernelGGL((conv3x3_fwd_kernel<float, __bf16, kFirst>), dim3(grid), dim3(256), 0, as_stream(s), nullptr,
+                     static_cast<__bf16*>(y), static_cast<const __bf16*>(wt), bias, N, H, W, 1, io);
+  return check_launch("conv3x3_first");
+}
+OPY_API int opy_conv3x3_last(const void* x, const float* f, int64_t stride, float* xo, float* yo, const void* wt,
+                             const float* bias, int N, int H, int W, int grid, void* s) {
+  if (!x || !f || !xo || !yo || !wt || N < 1 || H < 1 || W < 1 || grid < 1 || stride < (int64_t)H * W)
+    return fail(kInvalid, "conv3x3_last: bad arguments");
+  if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_last: input must be 16-byte aligned");
+  ConvIO io{f, stride, nullptr, xo, yo};
+  hipLaunchKernelGGL((conv3x3_fwd_kernel<__bf16, float, kLast>), dim3(grid), dim3(256), 0, as_stream(s),
+                     static_cast<const __bf16*>(x), nullptr, static_cast<const __bf16*>(wt), bias, N, H, W, 0, io);
+  return check_launch("conv3x3_last");
 }
 
 OPY_API int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,

@@ -31,6 +31,8 @@ def _bind():
     sigs = {
         "opy_conv3x3_fwd_bf16": [P, P, P, P, I, I, I, I, I, P],
         "opy_conv3x3_fwd_io": [P, P, P, P, I, I, I, I, I, I, I, P],
+        "opy_conv3x3_first": [P, L, P, P, P, P, I, I, I, I, P],
+        "opy_conv3x3_last": [P, P, L, P, P, P, P, I, I, I, I, P],
         "opy_conv3x3_wgrad_bf16": [P, P, P, I, I, I, I, P],
         "opy_sum_partials_f32": [P, I, I, P, P],
         "opy_conv3x3_prep_weights": [P, P, I, P],
@@ -159,6 +161,32 @@ class ConvNet(object):
             cur = out
         return (cur, acts) if keep else cur
 
+    def forward_packed(self, f_ptr, stride, loglam, N, H, W, xo_ptr, yo_ptr):
+        """Inference for N images of H x W at f_ptr + n*stride floats (the
+        solver's packed FP32 images) straight into the packed FP32 starts
+        x (xo_ptr + n*stride) and y (yo_ptr + 2n*stride): the same values as
+        forward(encoded input, bf16_acts=True) followed by the decoding
+        x = ((f - 0.5) + 0.5) + out0, y = (out1, out2), without the
+        32-channel FP32 input and output tensors.  loglam: (N,) float32
+        device tensor, the input's second channel."""
+        self._prep()
+        assert self.layers >= 2 and loglam.dtype == torch.float32 and loglam.numel() == N
+        s = _hip.stream()
+        key = (N, H, W, CH)
+        if key not in self._ibufs:
+            self._ibufs = {key: (torch.empty(key, dtype=torch.bfloat16, device="cuda"),
+                                 torch.empty(key, dtype=torch.bfloat16, device="cuda"),
+                                 torch.empty(key, dtype=torch.float32, device="cuda"))}
+        b16 = self._ibufs[key][:2]
+        _call("opy_conv3x3_first", f_ptr, int(stride), _hip.ptr(loglam), _hip.ptr(b16[0]), _hip.ptr(self._wf[0]),
+              _hip.ptr(self.b(0)), N, H, W, self.grid, s)
+        for l in range(1, self.layers - 1):
+            _call("opy_conv3x3_fwd_io", _hip.ptr(b16[(l - 1) % 2]), _hip.ptr(b16[l % 2]), _hip.ptr(self._wf[l]),
+                  _hip.ptr(self.b(l)), N, H, W, 1, 1, 1, self.grid, s)
+        l = self.layers - 1
+        _call("opy_conv3x3_last", _hip.ptr(b16[(l - 1) % 2]), f_ptr, int(stride), xo_ptr, yo_ptr,
+              _hip.ptr(self._wf[l]), _hip.ptr(self.b(l)), N, H, W, self.grid, s)
+
     def backward(self, acts, gout):
         """Gradients of all parameters into self.grads given dL/d(output)."""
         N, H, W, _ = gout.shape
@@ -245,13 +273,17 @@ class WarmStart(object):
         self.chunk = chunk
         self._inp = {}  # input tensors by shape: channels 2.. stay zero, 0 and 1 are rewritten per chunk
 
-    def predict_into(self, batch, lams):
+    def predict_into(self, batch, lams, fused=True):
         """Writes the prediction into batch.x[0] / batch.y[0] (the start buffers).
 
-        Instances of one shape go through the network in chunks; a chunk whose
-        instances sit at equal spacing in the packed buffers (consecutive
-        instances of one shape) is read and written through one strided view
-        per array, others instance by instance."""
+        Instances of one shape go through the network in chunks.  A chunk
+        whose instances sit at equal spacing in the packed buffers (consecutive
+        instances of one shape) of an FP32 batch goes through
+        ConvNet.forward_packed, whose first and last layers read the packed
+        images and write the packed starts themselves (fused=False: through
+        the encoded 32-channel tensors, as other chunks; the same values).
+        Other chunks are read and written through one strided view per array,
+        or instance by instance."""
         assert batch.C == 1
         f_all = batch.f
         groups = {}
@@ -265,11 +297,16 @@ class WarmStart(object):
                 offs = [batch.offs[b] for b in sel]
                 steps = {o1 - o0 for o0, o1 in zip(offs, offs[1:])}
                 stride = steps.pop() if len(steps) == 1 else (HW if n == 1 else None)
+                loglam = torch.as_tensor(np.log(np.asarray([lams[b] for b in sel], dtype=np.float64)) / 3.0,
+                                         dtype=torch.float32).cuda()
+                if fused and stride is not None and batch.dtype == torch.float32:
+                    self.net.forward_packed(_hip.ptr(f_all[offs[0]:]), stride, loglam, n, H, W,
+                                            _hip.ptr(batch.x[0][offs[0]:]), _hip.ptr(batch.y[0][2 * offs[0]:]))
+                    self._project(batch, offs, stride, HW)
+                    continue
                 inp = self._inp.get((n, H, W))
                 if inp is None:
                     inp = self._inp[(n, H, W)] = torch.zeros((n, H, W, CH), dtype=torch.float32, device="cuda")
-                loglam = torch.as_tensor(np.log(np.asarray([lams[b] for b in sel], dtype=np.float64)) / 3.0,
-                                         dtype=torch.float32).cuda()
                 inp[..., 1] = loglam[:, None, None]
                 if stride is not None:
                     fv = _rows(f_all, offs[0], stride, n, HW)
@@ -282,20 +319,20 @@ class WarmStart(object):
                 yv = out[..., 1:3].reshape(n, 2 * HW).to(batch.dtype)
                 if stride is not None:
                     _rows(batch.x[0], offs[0], stride, n, HW).copy_(xv)
-                    ydst = _rows(batch.y[0], 2 * offs[0], 2 * stride, n, 2 * HW)
-                    ydst.copy_(yv)
-                    # feasible dual start: project every pixel's pair onto |y| <= tvw
-                    if stride == HW:
-                        _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * offs[0]:2 * offs[0] + 2 * n * HW]),
-                                  n * HW, 1, 2, float(self.tvw), 0, _hip.stream())
-                    else:
-                        for off in offs:
-                            _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * off:2 * off + 2 * HW]), HW,
-                                      1, 2, float(self.tvw), 0, _hip.stream())
+                    _rows(batch.y[0], 2 * offs[0], 2 * stride, n, 2 * HW).copy_(yv)
                 else:
                     for j, off in enumerate(offs):
                         batch.x[0][off:off + HW].copy_(xv[j])
-                        ydst = batch.y[0][2 * off:2 * off + 2 * HW]
-                        ydst.copy_(yv[j])
-                        _hip.call("l1norms_proj", batch.dtype, _hip.ptr(ydst), HW, 1, 2, float(self.tvw), 0,
-                                  _hip.stream())
+                        batch.y[0][2 * off:2 * off + 2 * HW].copy_(yv[j])
+                self._project(batch, offs, stride, HW)
+
+    def _project(self, batch, offs, stride, HW):
+        """Feasible dual start: every pixel's pair of y onto |y| <= tvw."""
+        n = len(offs)
+        if stride == HW:
+            _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * offs[0]:2 * offs[0] + 2 * n * HW]),
+                      n * HW, 1, 2, float(self.tvw), 0, _hip.stream())
+        else:
+            for off in offs:
+                _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * off:2 * off + 2 * HW]), HW,
+                          1, 2, float(self.tvw), 0, _hip.stream())

@@ -183,3 +183,29 @@ def test_bf16_activations_give_the_same_output(gpu, shape):
     _call("opy_conv3x3_fwd_io", _hip.ptr(xd), _hip.ptr(y2), _hip.ptr(net._wf[0]), _hip.ptr(net.b(0)), N, H, W, 1,
           0, 0, net.grid, _hip.stream())
     np.testing.assert_array_equal(y1.cpu().numpy(), y2.cpu().numpy())
+
+
+@pytest.mark.parametrize("sizes", [[(40, 56)] * 5, [(17, 33), (40, 56), (17, 33), (64, 64), (17, 33)]])
+def test_fused_warm_start_io_equals_encoded_path(gpu, sizes):
+    """The warm start through opy_conv3x3_first / opy_conv3x3_last (input
+    encoding and output decoding inside the first and last layers) writes
+    bitwise the same starts as the path through the encoded 32-channel
+    tensors, for a chunk of equally spaced instances and for an uneven batch
+    (whose same-shape instances are not equally spaced)."""
+    import os
+    from opymize_amd.solvers.tv import TVBatch
+    from opymize_amd.tools import synth
+    from opymize_amd.warmstart.cnn import ConvNet, WarmStart
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "opymize_amd", "warmstart",
+                        "weights_tv_l2.npz")
+    imgs = [synth.rects(H, W, seed=50 + i) for i, (H, W) in enumerate(sizes)]
+    lams = [1.0, 2.5, 7.0, 13.0, 19.5][:len(imgs)]
+    ws = WarmStart(ConvNet.load(path), chunk=3)
+    out = {}
+    for fused in (False, True):
+        bat = TVBatch(imgs, lams, "l2", precision="float", tile=(64, 6))
+        ws.predict_into(bat, lams, fused=fused)
+        out[fused] = (bat.x[0].cpu().numpy(), bat.y[0].cpu().numpy())
+    np.testing.assert_array_equal(out[True][0], out[False][0])
+    np.testing.assert_array_equal(out[True][1], out[False][1])
+    assert np.abs(out[True][0]).max() > 0.1
