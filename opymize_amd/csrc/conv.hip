@@ -29,8 +29,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline __bf16 to_bf16(float v) { return (__bf16)v; }
 
-// The warm start's first and last layers read and write the solver's packed
-// buffers directly (image n of a group at n * stride elements):
+// The warm start's first and last layers (conv3x3_pipe_kernel) read and
+// write the solver's packed buffers directly (image n of a group at
+// n * stride elements):
 //   first (kFirst): input channel 0 = f - 0.5, channel 1 = loglam[n], the
 //   other 30 zero, i.e. the network's input encoding, formed while staging;
 //   last (kLast): x = ((f - 0.5) + 0.5) + o0 and y = (o1, o2) per pixel, the
@@ -50,12 +51,10 @@ struct ConvIO {
 // HBM (float, or __bf16 for activations between layers: the operands are
 // rounded to BF16 for the matrix cores anyway, so a BF16 copy of a layer's
 // output gives the next layer the same operands at half the traffic).
-// MODE: kPlain, or kFirst / kLast (x resp. y unused, see ConvIO).
-template <typename TX, typename TY, int MODE = kPlain>
+template <typename TX, typename TY>
 __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__ x, TY* __restrict__ y,
                                                           const __bf16* __restrict__ wt, const float* __restrict__ bias,
-                                                          int N, int H, int W, int act, ConvIO io = {}) {
-  constexpr int NJ = MODE == kLast ? 1 : 2;  // blocks of 16 output channels
+                                                          int N, int H, int W, int act) {
   __shared__ __align__(16) __bf16 xs[kHalo * kHalo * kPixStride];
   __shared__ __align__(16) __bf16 ws[9 * kCh * kPixStride];
   const int tid = threadIdx.x;
@@ -72,20 +71,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
     const int rem = t - n * tx * ty;
     const int h0 = (rem / tx) * kTile, w0 = (rem % tx) * kTile;
     __syncthreads();  // previous tile done with xs (and ws staged)
-    if constexpr (MODE == kFirst) {
-      const float* fb = io.f + (int64_t)n * io.stride;
-      const __bf16 ll = to_bf16(io.loglam[n]);
-      for (int e = tid; e < kHalo * kHalo * (kCh / 8); e += 256) {
-        const int q = e % (kCh / 8), p = e / (kCh / 8);
-        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
-        bf16x8 v = {};
-        if (q == 0 && hh >= 0 && hh < H && ww >= 0 && ww < W) {
-          v[0] = to_bf16(fb[(int64_t)hh * W + ww] - 0.5f);
-          v[1] = ll;
-        }
-        *reinterpret_cast<bf16x8*>(xs + p * kPixStride + 8 * q) = v;
-      }
-    } else if constexpr (sizeof(TX) == 4) {
+    if constexpr (sizeof(TX) == 4) {
       const TX* xb = x + (int64_t)n * H * W * kCh;
       for (int e = tid; e < kHalo * kHalo * (kCh / 4); e += 256) {
         const int q = e % (kCh / 4), p = e / (kCh / 4);
@@ -122,7 +108,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
       const int dy = tap / 3, dx = tap % 3;
       bf16x8 b[2];
 #pragma unroll
-      for (int j = 0; j < NJ; ++j)
+      for (int j = 0; j < 2; ++j)
         b[j] = *reinterpret_cast<const bf16x8*>(ws + (tap * kCh + j * 16 + li) * kPixStride + 8 * lk);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
@@ -130,11 +116,152 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
         const bf16x8 a =
             *reinterpret_cast<const bf16x8*>(xs + ((row + dy) * kHalo + li + dx) * kPixStride + 8 * lk);
 #pragma unroll
+        for (int j = 0; j < 2; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
+      }
+    }
+    TY* yb = y + (int64_t)n * H * W * kCh;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int h = h0 + wave * 4 + m;
+      if (h >= H) continue;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int co = j * 16 + li;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int w = w0 + 4 * lk + r;
+          if (w >= W) continue;
+          float v = acc[m][j][r] + bv;
+          if (act == 1) v = v > 0.f ? v : 0.f;
+          yb[((int64_t)h * W + w) * kCh + co] = (TY)v;
+        }
+      }
+    }
+  }
+}
+
+// ---- inference layers: BF16 activations, software-pipelined -----------------------
+// The arithmetic of conv3x3_fwd_kernel<__bf16, __bf16> (kPlain), and the
+// first and last layers reading and writing the solver's buffers (kFirst,
+// kLast; see ConvIO), for the warm start's inference: each block walks its
+// tiles with the next tile's
+// input already on its way into registers while the current one is on the
+// matrix cores, and a BF16 output tile is staged in LDS (in the input tile's
+// space, which the matrix products are done with) so that every pixel's 64
+// bytes leave in four 16-byte stores instead of 32-byte pieces.
+template <int MODE>
+__global__ void __launch_bounds__(256) conv3x3_pipe_kernel(const __bf16* __restrict__ x, __bf16* __restrict__ y,
+                                                           const __bf16* __restrict__ wt,
+                                                           const float* __restrict__ bias, int N, int H, int W, int act,
+                                                           ConvIO io) {
+  constexpr int NJ = MODE == kLast ? 1 : 2;  // blocks of 16 output channels
+  constexpr int NHP = kHalo * kHalo;         // pixels of a halo tile
+  constexpr int NV = NHP * (kCh / 8);        // 16-byte vectors of a halo tile
+  constexpr int NPF = MODE == kFirst ? (NHP + 255) / 256 : (NV + 255) / 256;
+  __shared__ __align__(16) __bf16 xs[NHP * kPixStride];
+  __shared__ __align__(16) __bf16 ws[9 * kCh * kPixStride];
+  static_assert(kTile * kTile <= NHP, "the output tile is staged in the input tile's space");
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 9 * kCh * kCh; e += 256) {
+    const int ci = e % kCh, rest = e / kCh;  // rest = tap*32 + co
+    ws[rest * kPixStride + ci] = wt[e];
+  }
+  const int tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
+  const int ntiles = N * tx * ty;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  auto origin = [&](int t, int& n, int& h0, int& w0) {
+    n = t / (tx * ty);
+    const int rem = t - n * tx * ty;
+    h0 = (rem / tx) * kTile;
+    w0 = (rem % tx) * kTile;
+  };
+  bf16x8 pv[MODE == kFirst ? 1 : NPF];  // the next tile's input vectors
+  float pf[MODE == kFirst ? NPF : 1];   // kFirst: the next tile's image values
+  auto prefetch = [&](int t) {
+    int n, h0, w0;
+    origin(t, n, h0, w0);
+    if constexpr (MODE == kFirst) {
+      const float* fb = io.f + (int64_t)n * io.stride;
+#pragma unroll
+      for (int i = 0; i < NPF; ++i) {
+        const int p = tid + 256 * i;
+        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+        pf[i] = (p < NHP && hh >= 0 && hh < H && ww >= 0 && ww < W) ? fb[(int64_t)hh * W + ww] : 0.f;
+      }
+    } else {
+      const __bf16* xb = x + (int64_t)n * H * W * kCh;
+#pragma unroll
+      for (int i = 0; i < NPF; ++i) {
+        const int e = tid + 256 * i;
+        const int q = e % (kCh / 8), p = e / (kCh / 8);
+        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+        pv[i] = bf16x8{};
+        if (e < NV && hh >= 0 && hh < H && ww >= 0 && ww < W)
+          pv[i] = *reinterpret_cast<const bf16x8*>(xb + ((int64_t)hh * W + ww) * kCh + 8 * q);
+      }
+    }
+  };
+  auto to_lds = [&](int t) {
+    int n, h0, w0;
+    origin(t, n, h0, w0);
+    if constexpr (MODE == kFirst) {
+      const __bf16 ll = to_bf16(io.loglam[n]);
+#pragma unroll
+      for (int i = 0; i < NPF; ++i) {
+        const int p = tid + 256 * i;
+        if (p >= NHP) continue;
+        const int hh = h0 - 1 + p / kHalo, ww = w0 - 1 + p % kHalo;
+        bf16x8 v = {};
+        if (hh >= 0 && hh < H && ww >= 0 && ww < W) {
+          v[0] = to_bf16(pf[i] - 0.5f);
+          v[1] = ll;
+        }
+        bf16x8* d = reinterpret_cast<bf16x8*>(xs + p * kPixStride);
+        d[0] = v;
+        d[1] = bf16x8{};
+        d[2] = bf16x8{};
+        d[3] = bf16x8{};
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NPF; ++i) {
+        const int e = tid + 256 * i;
+        if (e < NV) *reinterpret_cast<bf16x8*>(xs + (e / (kCh / 8)) * kPixStride + 8 * (e % (kCh / 8))) = pv[i];
+      }
+    }
+  };
+  if ((int)blockIdx.x < ntiles) prefetch(blockIdx.x);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's output has left xs (and ws is staged)
+    to_lds(t);
+    __syncthreads();
+    if (t + (int)gridDim.x < ntiles) prefetch(t + gridDim.x);
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;
+      bf16x8 b[2];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+        b[j] = *reinterpret_cast<const bf16x8*>(ws + (tap * kCh + j * 16 + li) * kPixStride + 8 * lk);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int row = wave * 4 + m;
+        const bf16x8 a =
+            *reinterpret_cast<const bf16x8*>(xs + ((row + dy) * kHalo + li + dx) * kPixStride + 8 * lk);
+#pragma unroll
         for (int j = 0; j < NJ; ++j) acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[m][j], 0, 0, 0);
       }
     }
+    int n, h0, w0;
+    origin(t, n, h0, w0);
     if constexpr (MODE == kLast) {
-      // decode: output channels 0..2 of the first block (lanes li < 3)
       const float* fb = io.f + (int64_t)n * io.stride;
       float* xo = io.xo + (int64_t)n * io.stride;
       float* yo = io.yo + 2 * (int64_t)n * io.stride;
@@ -154,25 +281,33 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const TX* __restrict__
           else yo[2 * p + li - 1] = v;
         }
       }
-      continue;
-    }
-    TY* yb = y + (int64_t)n * H * W * kCh;
+    } else {
+      __syncthreads();  // every wave is done with the input tile
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int h = h0 + wave * 4 + m;
-      if (h >= H) continue;
+      for (int m = 0; m < 4; ++m) {
+        const int row = wave * 4 + m;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int co = j * 16 + li;
-        const float bv = bias ? bias[co] : 0.f;
+        for (int j = 0; j < 2; ++j) {
+          const int co = j * 16 + li;
+          const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int w = w0 + 4 * lk + r;
-          if (w >= W) continue;
-          float v = acc[m][j][r] + bv;
-          if (act == 1) v = v > 0.f ? v : 0.f;
-          yb[((int64_t)h * W + w) * kCh + co] = (TY)v;
+          for (int r = 0; r < 4; ++r) {
+            float v = acc[m][j][r] + bv;
+            if (act == 1) v = v > 0.f ? v : 0.f;
+            xs[(row * kTile + 4 * lk + r) * kPixStride + co] = to_bf16(v);
+          }
         }
+      }
+      __syncthreads();
+      __bf16* yb = y + (int64_t)n * H * W * kCh;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = tid + 256 * i;
+        const int px = c / (kCh / 8), part = c % (kCh / 8);
+        const int h = h0 + px / kTile, w = w0 + px % kTile;
+        if (h < H && w < W)
+          *reinterpret_cast<bf16x8*>(yb + ((int64_t)h * W + w) * kCh + 8 * part) =
+              *reinterpret_cast<const bf16x8*>(xs + px * kPixStride + 8 * part);
       }
     }
   }
@@ -376,11 +511,13 @@ OPY_API int opy_conv3x3_fwd_io(const void* x, void* y, const void* wt, const flo
   if (!x || !y || !wt || N < 1 || H < 1 || W < 1 || grid < 1) return fail(kInvalid, "conv3x3_fwd_io: bad arguments");
   if (x == y) return fail(kInvalid, "conv3x3_fwd_io: in-place convolution is not supported");
   if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_fwd_io: input must be 16-byte aligned");
+  if (y_bf16 && (reinterpret_cast<uintptr_t>(y) & 15) != 0)
+    return fail(kInvalid, "conv3x3_fwd_io: BF16 output must be 16-byte aligned");
   const __bf16* w = static_cast<const __bf16*>(wt);
   hipStream_t st = as_stream(s);
   if (x_bf16 && y_bf16)
-    hipLaunchKernelGGL((conv3x3_fwd_kernel<__bf16, __bf16>), dim3(grid), dim3(256), 0, st,
-                       static_cast<const __bf16*>(x), static_cast<__bf16*>(y), w, bias, N, H, W, act);
+    hipLaunchKernelGGL((conv3x3_pipe_kernel<kPlain>), dim3(grid), dim3(256), 0, st, static_cast<const __bf16*>(x),
+                       static_cast<__bf16*>(y), w, bias, N, H, W, act, ConvIO{});
   else if (x_bf16)
     hipLaunchKernelGGL((conv3x3_fwd_kernel<__bf16, float>), dim3(grid), dim3(256), 0, st,
                        static_cast<const __bf16*>(x), static_cast<float*>(y), w, bias, N, H, W, act);
@@ -397,8 +534,9 @@ OPY_API int opy_conv3x3_first(const float* f, int64_t stride, const float* logla
                               const float* bias, int N, int H, int W, int grid, void* s) {
   if (!f || !loglam || !y || !wt || N < 1 || H < 1 || W < 1 || grid < 1 || stride < (int64_t)H * W)
     return fail(kInvalid, "conv3x3_first: bad arguments");
+  if ((reinterpret_cast<uintptr_t>(y) & 15) != 0) return fail(kInvalid, "conv3x3_first: output must be 16-byte aligned");
   ConvIO io{f, stride, loglam, nullptr, nullptr};
-  hipLaunchKernelGGL((conv3x3_fwd_kernel<float, __bf16, kFirst>), dim3(grid), dim3(256), 0, as_stream(s), nullptr,
+  hipLaunchKernelGGL((conv3x3_pipe_kernel<kFirst>), dim3(grid), dim3(256), 0, as_stream(s), nullptr,
                      static_cast<__bf16*>(y), static_cast<const __bf16*>(wt), bias, N, H, W, 1, io);
   return check_launch("conv3x3_first");
 }
@@ -408,7 +546,7 @@ OPY_API int opy_conv3x3_last(const void* x, const float* f, int64_t stride, floa
     return fail(kInvalid, "conv3x3_last: bad arguments");
   if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_last: input must be 16-byte aligned");
   ConvIO io{f, stride, nullptr, xo, yo};
-  hipLaunchKernelGGL((conv3x3_fwd_kernel<__bf16, float, kLast>), dim3(grid), dim3(256), 0, as_stream(s),
+  hipLaunchKernelGGL((conv3x3_pipe_kernel<kLast>), dim3(grid), dim3(256), 0, as_stream(s),
                      static_cast<const __bf16*>(x), nullptr, static_cast<const __bf16*>(wt), bias, N, H, W, 0, io);
   return check_launch("conv3x3_last");
 }
