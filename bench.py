@@ -208,7 +208,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=5)
