@@ -101,7 +101,18 @@ __global__ void __launch_bounds__(kTB) tv_restart_kernel(const opy_tv_inst_t* __
 }
 
 // ---- primal-dual gap (reference pdhg.py:23-44) ------------------------------
-// per tile partials: [P_data, P_tv, D_lin = sum z f, D_quad = sum z^2, max |y_p|, max |z|] with z = A^T y
+// per tile partials: [P_data, P_tv, D_lin = sum z f, D_quad = sum z^2, max |y_p|, max |z|] with z = A^T y.
+// A thread's rows go in groups of G (all of them for FP32 images of one
+// channel) whose loads are all issued before their arithmetic, and the six
+// partials leave the block in one LDS round (used by tile parity, so one
+// barrier per tile).
+template <typename T, int C, int RPT>
+constexpr int gap_group() {
+  int g = RPT;
+  while (g > 1 && (RPT % g != 0 || g * C * (int)sizeof(T) > 64)) --g;
+  return g;
+}
+
 template <typename T, int C, int RPT>
 __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, const T* __restrict__ y,
                                                      const T* __restrict__ f, const opy_tv_inst_t* __restrict__ insts,
@@ -109,13 +120,14 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
                                                      const int32_t* __restrict__ act_off,
                                                      const int32_t* __restrict__ nact_p, Cfg cfg,
                                                      double* __restrict__ partials) {
-  __shared__ double sm[kTB / 64];
+  __shared__ double sm[2 * (kTB / 64)][6];
+  int parity = 0;
   const int nact = *nact_p;
   if (nact <= 0) return;
   const int total = act_off[nact];
   const int tw = cfg.tw;
   const int th = (kTB / tw) * RPT;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tx = threadIdx.x % tw, ty = threadIdx.x / tw;
   const bool lhalo = (tx == 0) || (lane == 0);
   const bool rhalo = (tx == tw - 1) || (lane == 63);
@@ -129,68 +141,106 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
     const T* xb = x + I.off;
     const T* fb = f + I.off;
     const T* yb = y + 2 * I.off;
+    constexpr int G = gap_group<T, C, RPT>();
     double v[6] = {0, 0, 0, 0, 0, 0};
-    T xc[C], yup0[C];
+    T yup0[C];  // y0 of the row above the group
 #pragma unroll
-    for (int k = 0; k < C; ++k) xc[k] = yup0[k] = T(0);
-    if (col && rs < H) load_c<T, C>(xb + ((int64_t)rs * W + c) * C, xc);
+    for (int k = 0; k < C; ++k) yup0[k] = T(0);
     if (col && rs > 0 && rs < H) load_c<T, C>(yb + ((int64_t)(rs - 1) * W + c) * 2 * C, yup0);
 #pragma unroll
-    for (int i = 0; i < RPT; ++i) {
-      const int r = rs + i;
-      const bool ok = col && r < H;
-      const int64_t pix = (int64_t)r * W + c;
-      T xd[C], yv[2 * C], fv[C];
+    for (int g0 = 0; g0 < RPT; g0 += G) {
+      // ---- loads: x on the group's rows and the next (for the vertical
+      // difference), y and f on the group's rows; the halo lanes' right x and
+      // left y1
+      T xv[G + 1][C], yv[G][2 * C], fv[G][C], xr[G][C], yl1[G][C];
 #pragma unroll
-      for (int k = 0; k < C; ++k) xd[k] = fv[k] = yv[k] = yv[C + k] = T(0);
-      if (ok && r + 1 < H) load_c<T, C>(xb + (pix + W) * C, xd);
-      if (ok) {
-        load_c<T, 2 * C>(yb + pix * 2 * C, yv);
-        load_c<T, C>(fb + pix * C, fv);
+      for (int i = 0; i <= G; ++i)
+#pragma unroll
+        for (int k = 0; k < C; ++k) xv[i][k] = T(0);
+#pragma unroll
+      for (int i = 0; i < G; ++i)
+#pragma unroll
+        for (int k = 0; k < C; ++k) fv[i][k] = yv[i][k] = yv[i][C + k] = xr[i][k] = yl1[i][k] = T(0);
+#pragma unroll
+      for (int i = 0; i <= G; ++i) {
+        const int r = rs + g0 + i;
+        if (col && r < H) load_c<T, C>(xb + ((int64_t)r * W + c) * C, xv[i]);
       }
-      T xr[C], yl1[C];
 #pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xr[k] = shfl_down1(xc[k]);
-        yl1[k] = shfl_up1(yv[C + k]);
-      }
-      if (rhalo && ok && c + 1 < W) load_c<T, C>(xb + (pix + 1) * C, xr);
-      if (lhalo && ok && c > 0) load_c<T, C>(yb + (pix - 1) * 2 * C + C, yl1);
-      if (ok) {
-        T gx[2 * C], z[C];
-        grad_px<T, C>(xc, xd, xr, r, c, H, W, cfg, gx);
-        adjoint_px<T, C>(yv, yv + C, yup0, yl1, r, c, H, W, cfg, z);
-        double gn2 = 0.0, yn2 = 0.0;
-#pragma unroll
-        for (int e = 0; e < 2 * C; ++e) {
-          gn2 += (double)gx[e] * (double)gx[e];
-          yn2 += (double)yv[e] * (double)yv[e];
+      for (int i = 0; i < G; ++i) {
+        const int r = rs + g0 + i;
+        if (col && r < H) {
+          const int64_t pix = (int64_t)r * W + c;
+          load_c<T, 2 * C>(yb + pix * 2 * C, yv[i]);
+          load_c<T, C>(fb + pix * C, fv[i]);
+          if (rhalo && c + 1 < W) load_c<T, C>(xb + (pix + 1) * C, xr[i]);
+          if (lhalo && c > 0) load_c<T, C>(yb + (pix - 1) * 2 * C + C, yl1[i]);
         }
-        v[1] += sqrt(gn2);
-        v[4] = fmax(v[4], sqrt(yn2));
+      }
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int r = rs + g0 + i;
+        const bool ok = col && r < H;
+        T xri[C], yli[C];
 #pragma unroll
         for (int k = 0; k < C; ++k) {
-          const double d = (double)xc[k] - (double)fv[k];
-          v[0] += I.data == OPY_TV_DATA_L1 ? fabs(d) : 0.5 * d * d;
-          const double zk = (double)z[k];
-          v[2] += zk * (double)fv[k];
-          v[3] += zk * zk;
-          v[5] = fmax(v[5], fabs(zk));
+          const T sr = shfl_down1(xv[i][k]);
+          const T sl = shfl_up1(yv[i][C + k]);
+          xri[k] = rhalo ? xr[i][k] : sr;
+          yli[k] = lhalo ? yl1[i][k] : sl;
+        }
+        if (ok) {
+          T gx[2 * C], z[C];
+          grad_px<T, C>(xv[i], xv[i + 1], xri, r, c, H, W, cfg, gx);
+          adjoint_px<T, C>(yv[i], yv[i] + C, i == 0 ? yup0 : yv[i - 1], yli, r, c, H, W, cfg, z);
+          double gn2 = 0.0, yn2 = 0.0;
+#pragma unroll
+          for (int e = 0; e < 2 * C; ++e) {
+            gn2 += (double)gx[e] * (double)gx[e];
+            yn2 += (double)yv[i][e] * (double)yv[i][e];
+          }
+          v[1] += sqrt(gn2);
+          v[4] = fmax(v[4], sqrt(yn2));
+#pragma unroll
+          for (int k = 0; k < C; ++k) {
+            const double d = (double)xv[i][k] - (double)fv[i][k];
+            v[0] += I.data == OPY_TV_DATA_L1 ? fabs(d) : 0.5 * d * d;
+            const double zk = (double)z[k];
+            v[2] += zk * (double)fv[i][k];
+            v[3] += zk * zk;
+            v[5] = fmax(v[5], fabs(zk));
+          }
         }
       }
 #pragma unroll
-      for (int k = 0; k < C; ++k) {
-        xc[k] = xd[k];
-        yup0[k] = yv[k];
-      }
+      for (int k = 0; k < C; ++k) yup0[k] = yv[G - 1][k];
     }
-    for (int j = 0; j < 4; ++j) v[j] = block_sum(v[j], sm);
-    v[4] = block_max(v[4], sm);
-    v[5] = block_max(v[5], sm);
+    // ---- the six partials of the tile: wavefront shuffles, one LDS round
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] += __shfl_down(v[j], off, 64);
+#pragma unroll
+      for (int j = 4; j < 6; ++j) v[j] = fmax(v[j], __shfl_down(v[j], off, 64));
+    }
+    double* buf = sm[parity * (kTB / 64) + wave];
+    if (lane == 0)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) buf[j] = v[j];
+    __syncthreads();
     if (threadIdx.x == 0) {
       double* p = partials + 6 * (int64_t)(I.tile0 + tr.j);
-      for (int j = 0; j < 6; ++j) p[j] = v[j];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double a = sm[parity * (kTB / 64)][j];
+        for (int w = 1; w < kTB / 64; ++w) {
+          const double b = sm[parity * (kTB / 64) + w][j];
+          a = j < 4 ? a + b : fmax(a, b);
+        }
+        p[j] = a;
+      }
     }
+    parity ^= 1;
   }
 }
 
