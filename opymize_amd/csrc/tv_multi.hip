@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
   // main ones (j < NR) are rows wave*NR + j of column K + lane (rows beyond
   // the region up to 4 NR - 1: never loaded or stored), i.e. an LDS index
   // that grows by PW per row; the halo ones are listed (row -1 = none).
-  const int mrow = wave * NR;
+  const int mrow = __builtin_amdgcn_readfirstlane(wave * NR);  // uniform over the wavefront
   const int mq = (mrow + 1) * PW + K + lane + 1;
   int hr[NH], hcol[NH], hq[NH];
 #pragma unroll
