@@ -435,7 +435,7 @@ class TVBatch(object):
                 st[b, F["tau"]] = st[b, F["sigma"]] = np.sqrt(step_bound)
                 st[b, F["alpha"]] = 0.5
             else:
-                st[b, F["sigma"]] = np.sqrt(step_bound / step_factor)
+                st[b, F["sigma"]] = np.sqrt(step_bound / np.broadcast_to(step_factor, (self.B,))[b])
                 st[b, F["tau"]] = step_bound / st[b, F["sigma"]]
         self.state.copy_(torch.as_tensor(st))
         rec = self.restarts.cpu().numpy()
@@ -448,7 +448,8 @@ class TVBatch(object):
 
     def solve(self, steps="adaptive", step_bound=0.124, step_factor=1.0, tol=1e-5, tol_infeas=None,
               max_iters=20000, block_iters=50, check_every=1, certify=True, use_graph=True, eps_res=None,
-              callback=None, check_start=False, keep_state=False, fallback=False, restart="auto"):
+              callback=None, check_start=False, keep_state=False, fallback=False, restart="auto",
+              step_factor_l1=None):
         """Iterate until every instance is certified (|relgap| < tol and
         infeasibility < tol_infeas) or max_iters; the test runs every
         block_iters*check_every iterations.  Returns self.results().
@@ -463,7 +464,14 @@ class TVBatch(object):
         restart, its certified relative gap has fallen below beta times the
         gap at that restart or has risen above the smallest gap since;
         (beta, min_iters) sets the two, RESTART_DEFAULT otherwise; False
-        or None keeps the plain schedule."""
+        or None keeps the plain schedule.
+
+        step_factor_l1 (accelerated steps): the initial split sigma =
+        sqrt(step_bound / f), tau = step_bound / sigma of the TV-L1
+        instances, which keep constant steps under this rule (step_factor
+        otherwise).  Small values suit them: on the uneven 512-instance
+        batch, 1/256 certifies 267 of 276 TV-L1 instances in a median of 48
+        iterations, against 218 in a median of 864 at 4."""
         if steps not in STEP_MODES:
             raise ValueError("steps must be one of %s" % sorted(STEP_MODES))
         tol_infeas = tol if tol_infeas is None else tol_infeas
@@ -477,12 +485,16 @@ class TVBatch(object):
         e = tol if eps_res is None else eps_res
         params.eta, params.Delta = 0.95, 1.5
         params.eps_absp = params.eps_relp = params.eps_absd = params.eps_reld = e
+        restart_factor = step_factor
+        if steps == "accel" and step_factor_l1 is not None:
+            step_factor = np.where(np.array([d == "l1" for d in self.data]), float(step_factor_l1),
+                                   float(step_factor))
         if not keep_state:
             self._reset_state(steps, step_bound, step_factor, tol, tol_infeas, certify)
         self._restart = None
         if steps == "accel" and restart not in (False, None):
             beta, min_iters = RESTART_DEFAULT if restart in ("auto", True) else restart
-            sigma0 = np.sqrt(step_bound / step_factor)
+            sigma0 = np.sqrt(step_bound / restart_factor)  # TV-L2 instances only restart
             self._restart = (float(beta), int(min_iters), step_bound / sigma0, sigma0)
         self.cur = 0
         if check_start:

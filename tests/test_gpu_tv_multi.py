@@ -170,6 +170,30 @@ def test_restarts_match_oracle(gpu, multi):
     assert total > 0
 
 
+def test_tv_l1_split_under_accelerated_steps(gpu):
+    """step_factor_l1: under the accelerated rule the TV-L1 instances start
+    from their own split (and keep constant steps), the TV-L2 ones from
+    step_factor; FP64 iterates against the oracle with each instance's
+    factor (its accelerated rule reduces to constant steps at gamma = 0)."""
+    bat, imgs, lams, data = _batch("double", (64, 2), 4)
+    k = 40
+    bat.solve(steps="accel", step_bound=0.124, step_factor=4.0, step_factor_l1=1.0 / 256, max_iters=k,
+              block_iters=20, tol=1e-30, restart=False)
+    xs, ys = bat.solution()
+    st = bat.steps_state()
+    assert "l1" in data and "l2" in data
+    for b, im in enumerate(imgs):
+        H, W = im.shape
+        fac = 1.0 / 256 if data[b] == "l1" else 4.0
+        prob = opd.tv_problem(im.reshape(-1, 1), (H, W), lams[b], data[b])
+        xo, yo, io, _ = opd.solve(prob, steps="accel", step_bound=0.124, step_factor=fac, term_maxiter=k,
+                                  granularity=k, term_pd_gap=1e-30)
+        assert np.abs(xs[b] - xo).max() < 1e-9, b
+        assert np.abs(ys[b] - yo).max() < 1e-9, b
+        if data[b] == "l1":
+            assert abs(st[b]["sigma"] - np.sqrt(0.124 * 256)) < 1e-12 * np.sqrt(0.124 * 256)
+
+
 def test_restarts_certify_a_stalling_instance(gpu):
     """FP32, accelerated steps: the instance with lambda = 1.29 of this batch
     does not reach a certified relative gap of 1e-5 in 20000 iterations
