@@ -438,11 +438,12 @@ class TVBatch(object):
                 st[b, F["sigma"]] = np.sqrt(step_bound / np.broadcast_to(step_factor, (self.B,))[b])
                 st[b, F["tau"]] = step_bound / st[b, F["sigma"]]
         self.state.copy_(torch.as_tensor(st))
+        # restart records (TVRestart: ref, best, last | count): none yet, last = now
         rec = self.restarts.cpu().numpy()
         iters = self.checks.cpu().numpy().view(np.int32)[:, 17]
         for b in np.flatnonzero(flags):
             rec[b, :2] = np.inf
-            rec[b:b + 1, 2:3].view(np.int32)[0, 0] = iters[b]
+            rec.view(np.int32)[b, 4] = iters[b]
         self.restarts.copy_(torch.as_tensor(rec))
         self._compact(0)
 

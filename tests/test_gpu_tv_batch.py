@@ -174,3 +174,30 @@ def test_accelerated_steps_match_oracle(gpu, precision, passes, tile):
         assert np.abs(ys[b] - yo).max() < tol, b
         assert abs(st[b]["tau"] - io["tau"]) <= 1e-12 * io["tau"], b
         assert abs(st[b]["sigma"] - io["sigma"]) <= 1e-12 * io["sigma"], b
+
+
+def test_warm_start_fallback_restarts_cold(gpu):
+    """fallback=True: an instance whose start is not finite is restarted
+    cold (x = y = 0, fresh steps and restart record) at the first test and
+    then certifies like the others; the instances with a usable start keep
+    it.  FP64, accelerated steps with restarts and the TV-L1 split."""
+    bat, imgs, lams, data = make_batch("double")
+    ref, *_ = make_batch("double")
+    ref.solve(steps="accel", step_bound=0.124, step_factor=4.0, step_factor_l1=1.0 / 256, max_iters=20000,
+              block_iters=40, tol=1e-7)
+    x0 = bat.pack([im for im in imgs], "x")  # x = f: a usable start
+    bad = 2
+    x0[bat.offs[bad]:bat.offs[bad] + imgs[bad].size] = np.nan
+    bat.set_start(x0, None)
+    res = bat.solve(steps="accel", step_bound=0.124, step_factor=4.0, step_factor_l1=1.0 / 256, max_iters=20000,
+                    block_iters=40, tol=1e-7, fallback=True)
+    assert list(bat.fallbacks) == [bad]
+    assert all(r["status"] == 1 for r in res), res
+    xs, _ = bat.solution()
+    xr, _ = ref.solution()
+    rr = ref.results()
+    for b, lam in enumerate(lams):
+        if data[b] != "l2":
+            continue
+        bound = np.sqrt(2 * max(res[b]["gap"], 0) / lam) + np.sqrt(2 * max(rr[b]["gap"], 0) / lam)
+        assert np.linalg.norm(xs[b] - xr[b]) <= 1.01 * bound + 1e-6, b
