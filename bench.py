@@ -233,8 +233,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (batches would shard across GPUs by "
-              "instance, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: opymize_amd has no multi-GPU path (a batch would shard across GPUs by "
+              "instance, each GPU solving its own share); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
