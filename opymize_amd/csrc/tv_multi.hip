@@ -129,13 +129,9 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
         load_c<T, C>(fb + p * C, fv[j]);
       }
     }
-#pragma unroll
-    for (int j = 0; j < NS; ++j)
-      if (j < NR || srow(j) >= 0)
-#pragma unroll
-        for (int e = 0; e < 2 * C; ++e) sy[e][sidx(j)] = yv[j][e];
     // the K iterations' scalar steps, formed once per tile (FP64, as K single
-    // launches and opy_tv_accel would form them) instead of in every wave
+    // launches and opy_tv_accel would form them) instead of in every wave,
+    // while the loads are in flight
     if (!PC && threadIdx.x == 0) {
       double tau_d = st[tr.b].tau, sig_d = st[tr.b].sigma;
       for (int i = 0; i < K; ++i) {
@@ -145,6 +141,11 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
         sig_d /= th_d;
       }
     }
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+      if (j < NR || srow(j) >= 0)
+#pragma unroll
+        for (int e = 0; e < 2 * C; ++e) sy[e][sidx(j)] = yv[j][e];
     __syncthreads();
     const T tvw = (T)I.tvw;
     // preconditioned steps do not change between iterations (theta = 1)
