@@ -90,15 +90,22 @@ __device__ inline int xcd_block() {
 }
 
 // tile g of the running instances -> (instance, tile index, first row, first
-// column): binary search of g in act_off; tiles are tw image columns by th
-// rows, numbered row by row within an instance
+// column): search of g in act_off; tiles are tw image columns by th rows,
+// numbered row by row within an instance
 __device__ inline TileRef locate(int g, const int32_t* __restrict__ active_inst, const int32_t* __restrict__ act_off,
                                  int nact, const opy_tv_inst_t* __restrict__ insts, int tw, int th) {
-  int lo = 0, hi = nact - 1;  // largest a with act_off[a] <= g
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (act_off[mid] <= g) lo = mid;
-    else hi = mid - 1;
+  // largest a with act_off[a] <= g: first the proportional guess (exact when
+  // the running instances have equal tile counts; its two loads are
+  // independent), else a binary search (dependent loads)
+  int lo = min((int)((int64_t)g * nact / act_off[nact]), nact - 1);
+  if (!(act_off[lo] <= g && g < act_off[lo + 1])) {
+    int hi = nact - 1;
+    lo = 0;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (act_off[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
   }
   TileRef t;
   t.b = active_inst[lo];
@@ -154,15 +161,6 @@ struct PrimalStep {
   __device__ PrimalStep(double tau_s, const opy_tv_inst_t& I, const Cfg& cfg) : lam(I.lam), data(I.data) {
 #pragma unroll
     for (int k = 0; k < C; ++k) coef(PC ? precond_tau<T>(1, 1, 3, 3, k, cfg) : (T)tau_s, tau[k], a[k], b[k]);
-  }
-  // scalar steps from coefficients formed elsewhere by coef()
-  __device__ PrimalStep(T t, T ak, T bk, const opy_tv_inst_t& I) : lam(I.lam), data(I.data) {
-#pragma unroll
-    for (int k = 0; k < C; ++k) {
-      tau[k] = t;
-      a[k] = ak;
-      b[k] = bk;
-    }
   }
   __device__ void coef(T t, T& tk, T& ak, T& bk) const {
     const double tl = (double)t * lam;
