@@ -45,10 +45,11 @@ constexpr int kMultiLdsMax = 64 * 1024;
 // Threads: the 64 tile columns of every region row belong to one lane each
 // (rows split over the 4 waves; aligned vector loads and stores), and the
 // 2K halo columns x region rows are spread one pixel per thread (two with
-// large tiles).  Each thread keeps x, f and y of its pixels in registers;
-// LDS holds y and xbar, the values its neighbours read.  Two barriers per
-// iteration: after xbar is written (phase A) and after y is written
-// (phase B).
+// large tiles), row by row, so that a wavefront's halo loads touch a few
+// short row segments instead of 64 rows of one column.  Each thread keeps
+// x, f and y of its pixels in registers; LDS holds y and xbar, the values
+// its neighbours read.  Two barriers per iteration: after xbar is written
+// (phase A) and after y is written (phase B).
 //
 // Steps: tau, sigma (and theta with accelerated steps) of iteration i are
 // derived from the instance's state exactly as the single-step launches and
@@ -91,9 +92,9 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
   int hr[NH], hcol[NH], hq[NH];
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    const int t = threadIdx.x + h * kTB;
-    const int hc = t / RH;
-    hr[h] = t < NHALO ? t - hc * RH : -1;
+    const int t = threadIdx.x + h * kTB;  // halo pixels row by row: 2K per region row
+    const int hc = t % (2 * K);
+    hr[h] = t < NHALO ? t / (2 * K) : -1;
     hcol[h] = hc < K ? hc : 64 + hc;
     hq[h] = (hr[h] + 1) * PW + hcol[h] + 1;
   }
