@@ -105,7 +105,8 @@ def test_training_reduces_loss(gpu):
     k = np.ones((3, 3)) / 9.0
     from scipy.signal import correlate
     for n in range(N):
-        tgt[n, ..., 0] = correlate(x[n, ..., 0], k, mode="same") - x[n, ..., 0]
+        xn = x[n, ..., 0].astype(np.float64)
+        tgt[n, ..., 0] = correlate(xn, k, mode="same") - xn
     xd, td = torch.as_tensor(x).cuda(), torch.as_tensor(tgt).cuda()
     losses = [net.train_step(xd, td, lr=3e-3).sum().item() for _ in range(300)]
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
