@@ -142,6 +142,8 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
     const T* fb = f + I.off;
     const T* yb = y + 2 * I.off;
     constexpr int G = gap_group<T, C, RPT>();
+    // the tile with a pixel to spare on every side inside the image
+    const bool inner = tr.r0 >= 1 && tr.r0 + th < H && tr.c0 >= 1 && tr.c0 + tw < W;
     double v[6] = {0, 0, 0, 0, 0, 0};
     T yup0[C];  // y0 of the row above the group
 #pragma unroll
@@ -189,10 +191,12 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
           xri[k] = rhalo ? xr[i][k] : sr;
           yli[k] = lhalo ? yl1[i][k] : sl;
         }
-        if (ok) {
+        // M: boundary masks of the stencils (none on interior tiles)
+        auto pixel = [&](auto m_tag) {
+          constexpr bool M = decltype(m_tag)::value;
           T gx[2 * C], z[C];
-          grad_px<T, C>(xv[i], xv[i + 1], xri, r, c, H, W, cfg, gx);
-          adjoint_px<T, C>(yv[i], yv[i] + C, i == 0 ? yup0 : yv[i - 1], yli, r, c, H, W, cfg, z);
+          grad_px<T, C, M>(xv[i], xv[i + 1], xri, r, c, H, W, cfg, gx);
+          adjoint_px<T, C, M>(yv[i], yv[i] + C, i == 0 ? yup0 : yv[i - 1], yli, r, c, H, W, cfg, z);
           double gn2 = 0.0, yn2 = 0.0;
 #pragma unroll
           for (int e = 0; e < 2 * C; ++e) {
@@ -200,7 +204,7 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
             yn2 += (double)yv[i][e] * (double)yv[i][e];
           }
           v[1] += sqrt(gn2);
-          v[4] = fmax(v[4], sqrt(yn2));
+          v[4] = fmax(v[4], yn2);  // |y_p|^2: the square root of the maximum is taken per tile
 #pragma unroll
           for (int k = 0; k < C; ++k) {
             const double d = (double)xv[i][k] - (double)fv[i][k];
@@ -210,6 +214,10 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
             v[3] += zk * zk;
             v[5] = fmax(v[5], fabs(zk));
           }
+        };
+        if (ok) {
+          if (inner) pixel(std::false_type{});
+          else pixel(std::true_type{});
         }
       }
 #pragma unroll
@@ -237,7 +245,7 @@ __global__ void __launch_bounds__(kTB) tv_gap_kernel(const T* __restrict__ x, co
           const double b = sm[parity * (kTB / 64) + w][j];
           a = j < 4 ? a + b : fmax(a, b);
         }
-        p[j] = a;
+        p[j] = j == 4 ? sqrt(a) : a;  // max |y_p| = sqrt(max |y_p|^2) (sqrt is monotone and correctly rounded)
       }
     }
     parity ^= 1;
