@@ -475,6 +475,11 @@ class TVBatch(object):
         iterations, against 218 in a median of 864 at 4."""
         if steps not in STEP_MODES:
             raise ValueError("steps must be one of %s" % sorted(STEP_MODES))
+        if restart not in ("auto", True, False, None):
+            if len(restart) != 2 or not 0.0 <= float(restart[0]) < 1.0 or int(restart[1]) < 0:
+                raise ValueError("restart must be 'auto', True, False, None or (beta in [0, 1), min_iters >= 0)")
+        if step_factor_l1 is not None and not float(step_factor_l1) > 0.0:
+            raise ValueError("step_factor_l1 must be positive")
         tol_infeas = tol if tol_infeas is None else tol_infeas
         if fallback:
             check_start = True

@@ -220,6 +220,11 @@ def test_multi_rejects_bad_arguments(gpu):
     bat.set_multi(2)  # rows per thread 12 is a single-pass tile, not a multi-iteration one
     with pytest.raises(_hip.HipError, match="rows per thread"):
         bat.solve(steps="const", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30)
+    for bad in ((1.0, 96), (0.5, -1), (0.5,)):
+        with pytest.raises(ValueError, match="restart"):
+            bat.solve(steps="accel", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30, restart=bad)
+    with pytest.raises(ValueError, match="step_factor_l1"):
+        bat.solve(steps="accel", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30, step_factor_l1=0.0)
 
 
 def test_autotune_considers_iterations_per_launch(gpu):
