@@ -14,7 +14,9 @@
   instances against their oracle runs (FP32 5e-4), every gap finite.
 - The learned-warm-start batch (256 x 1024^2 TV-L2): 48 iterations with
   4 iterations per launch against single-iteration launches (1e-6), and
-  instance 0 against the oracle (5e-4).
+  instance 0 against the oracle (5e-4); and its smallest-lambda instance in
+  FP64 with restarts of the accelerated schedule, 240 iterations against the
+  oracle's restart rule (same restarts, 1e-9).
 """
 import numpy as np
 import pytest
@@ -134,3 +136,30 @@ def test_warm_start_batch_256x1024(gpu):
     e = np.abs(out[4][2] - xo).max()
     print("256 x 1024^2, %d accelerated iterations, 4 per launch: max |x - x_oracle| on instance 0 = %.2e" % (k, e))
     assert e < 5e-4
+
+
+def test_restarts_at_full_size_1024(gpu):
+    """The bench's solver on one 1024^2 instance of its batch (the smallest
+    lambda) in FP64: 4 iterations per launch on 64 x 24 tiles, accelerated
+    steps with restarts (tests every 24 iterations, decay 0.5, at least 48
+    iterations apart), 240 iterations; the same restarts and iterates as the
+    oracle's rule at full size (1e-9)."""
+    from opymize_amd.solvers.tv import TVBatch
+    from opymize_amd.tools import synth
+    specs = synth.batch_square(256, side=1024, seed=5)
+    b = int(np.argmin([s["lam"] for s in specs]))
+    img = synth.images([specs[b]])[0]
+    lam = specs[b]["lam"]
+    k = 240
+    bat = TVBatch([img], [lam], "l2", precision="double", tile=(64, 6), multi=4)
+    res = bat.solve(steps="accel", step_bound=0.124, step_factor=4.0, max_iters=k, block_iters=24, tol=1e-30,
+                    restart=(0.5, 48))
+    xs, ys = bat.solution()
+    xo, yo, io, _ = opd.solve(opd.tv_problem(img.reshape(-1, 1), (1024, 1024), lam), steps="accel",
+                              step_bound=0.124, step_factor=4.0, term_maxiter=k, granularity=24, term_pd_gap=1e-30,
+                              restart=(0.5, 48))
+    e = max(np.abs(xs[0] - xo).max(), np.abs(ys[0] - yo).max())
+    print("1024^2, lambda %.3f, %d iterations: restarts GPU %d, oracle %s; max |GPU - oracle| = %.2e"
+          % (lam, k, res[0]["restarts"], io["restarts"], e))
+    assert res[0]["restarts"] == len(io["restarts"]) and res[0]["restarts"] > 0
+    assert e < 1e-9
