@@ -1,4 +1,6 @@
 """Linear operators (reference: opymize/linear/__init__.py)."""
+import logging
+
 import numpy as np
 import torch
 
@@ -60,7 +62,7 @@ def normest(linop, xy=None, tol=1.0e-4, maxits=500, dtype=torch.float64):
         _hip.call("scale", x.dtype, _hip.ptr(x), _hip.ptr(x), n, None, 1.0 / normx, 0, _hip.stream())
         itn += 1
         if itn > maxits:
-            print("Warning: normest didn't converge!")
+            logging.warning("normest: no convergence within %d power iterations (estimate %g)", maxits, e)
             break
     return e, itn
 
@@ -90,10 +92,13 @@ class BlockOp(LinOp):
         self.blocks = blocks
         for i, Bi in enumerate(self.blocks):
             for j, Bij in enumerate(Bi):
+                # a mismatch would make the block kernels read or write past a view
                 if Bij.y.size != self.blocks[i][0].y.size:
-                    print("check_blockdims: y-problem in (%d,%d)" % (i, j))
+                    raise ValueError("BlockOp: block (%d,%d) has output size %d, row %d expects %d"
+                                     % (i, j, Bij.y.size, i, self.blocks[i][0].y.size))
                 if Bij.x.size != self.blocks[0][j].x.size:
-                    print("check_blockdims: x-problem in (%d,%d)" % (i, j))
+                    raise ValueError("BlockOp: block (%d,%d) has input size %d, column %d expects %d"
+                                     % (i, j, Bij.x.size, j, self.blocks[0][j].x.size))
         if adjoint is None:
             adj_bl = [[b[j].adjoint for b in self.blocks] for j in range(len(blocks[0]))]
             self.adjoint = BlockOp(adj_bl, adjoint=self)

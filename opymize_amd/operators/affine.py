@@ -48,6 +48,8 @@ class ConstOp(Operator):
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
         y = _out(x, y)
+        if y.numel() != self.const.size:
+            raise ValueError("ConstOp: output has %d entries, the constant %d" % (y.numel(), self.const.size))
         c = self._const.get("c", self.const.ravel(), y.dtype)
         _hip.call("const", y.dtype, _hip.ptr(y), y.numel(), _hip.ptr(c), int(add), _hip.stream())
         if jacobian:
@@ -70,6 +72,11 @@ class ConstrainOp(Operator):
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
         y = _out(x, y)
+        if x.numel() != self.const.size or y.numel() != self.const.size:
+            raise ValueError("ConstrainOp: input/output have %d/%d entries, the constant %d"
+                             % (x.numel(), y.numel(), self.const.size))
+        if self.mask.size != self.const.shape[0]:
+            raise ValueError("ConstrainOp: mask has %d entries for %d rows" % (self.mask.size, self.const.shape[0]))
         M = self.const.shape[1]
         c = self._const.get("c", self.const.ravel(), x.dtype)
         m = self._const.get_u8("mask", self.mask.ravel())
@@ -93,7 +100,12 @@ class ShiftScaleOp(Operator):
         self._jacobian = ScaleOp(N, self.a)
         self._const = DeviceConstants()
 
-    def _arg(self, name, v, dtype):
+    def _arg(self, name, v, dtype, n):
+        if isinstance(v, np.ndarray) and v.size == 1 and n != 1:
+            v = float(v.ravel()[0])
+        if isinstance(v, (np.ndarray, torch.Tensor)) and (v.size if isinstance(v, np.ndarray) else v.numel()) != n:
+            raise ValueError("ShiftScaleOp: %s has %d entries, the input %d"
+                             % (name, v.size if isinstance(v, np.ndarray) else v.numel(), n))
         if isinstance(v, np.ndarray):
             key = "%s:%d" % (name, id(v))
             if getattr(self, "_id_" + name, None) != key:
@@ -106,11 +118,14 @@ class ShiftScaleOp(Operator):
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
         y = _out(x, y)
-        sa, ss = self._arg("shift", self.shift, x.dtype)
-        aa, as_ = self._arg("a", self.a, x.dtype)
-        ba, bs = self._arg("b", self.b, x.dtype)
-        b_zero = not isinstance(self.b, (np.ndarray, torch.Tensor)) and float(self.b) == 0
-        s_zero = not isinstance(self.shift, (np.ndarray, torch.Tensor)) and float(self.shift) == 0
+        if y.numel() != x.numel():
+            raise ValueError("ShiftScaleOp: input has %d entries, output %d" % (x.numel(), y.numel()))
+        n = x.numel()
+        sa, ss = self._arg("shift", self.shift, x.dtype, n)
+        aa, as_ = self._arg("a", self.a, x.dtype, n)
+        ba, bs = self._arg("b", self.b, x.dtype, n)
+        b_zero = bs == 0 and ba is None
+        s_zero = ss == 0 and sa is None
         if b_zero or s_zero:
             ba, bs = None, 0.0
         _hip.call("shiftscale", x.dtype, _hip.ptr(x), _hip.ptr(y), x.numel(), sa, ss, aa, as_, ba, bs, int(add),
