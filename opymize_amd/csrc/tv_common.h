@@ -479,6 +479,8 @@ int check_cfg(const opy_tv_cfg_t* c) {
   if (c->tw < 1 || c->tw > kTB || kTB % c->tw) return fail(kInvalid, "tv: tile width %d must divide %d", c->tw, kTB);
   if (c->rpt != 1 && c->rpt != 2 && c->rpt != 4 && c->rpt != 6 && c->rpt != 8 && c->rpt != 12 && c->rpt != 16)
     return fail(kInvalid, "tv: rows per thread %d not in {1,2,4,6,8,12,16}", c->rpt);
+  if (c->C == 4 && c->rpt == 16)
+    return fail(kInvalid, "tv: C=4 with 16 rows per thread exceeds the register budget (at most 12)");
   if (c->steps != OPY_TV_STEPS_SCALAR && c->steps != OPY_TV_STEPS_PRECOND && c->steps != OPY_TV_STEPS_ACCEL)
     return fail(kInvalid, "tv: bad step mode");
   if (!(c->h0 > 0) || !(c->h1 > 0)) return fail(kInvalid, "tv: grid spacing must be positive");

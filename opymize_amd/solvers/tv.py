@@ -184,6 +184,11 @@ class TVBatch(object):
     def set_tile(self, tw, rpt):
         if self.passes == 1:
             tw = 64
+        if int(rpt) not in (1, 2, 4, 6, 8, 12, 16):
+            raise ValueError("rows per thread %d not in {1, 2, 4, 6, 8, 12, 16}" % rpt)
+        if self.C == 4 and int(rpt) == 16:
+            raise ValueError("C=4 with 16 rows per thread exceeds the fused kernels' register budget; "
+                             "use at most 12 rows per thread")
         th = tile_height(tw, rpt)
         self.tw, self.rpt = int(tw), int(rpt)
         insts = (TVInst * self.B)()

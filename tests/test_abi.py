@@ -61,3 +61,22 @@ def test_invalid_arguments_fail_without_gpu_work():
     assert rc == 1
     rc = lib.opy_quadepi_proj_f64(None, 10, 99, 1.0, None, float("inf"), None)
     assert rc == 1 and b"M=99" in lib.opy_last_error()
+
+
+def test_tv_entry_points_refuse_c4_with_16_rows():
+    """The fused TV entry points check their configuration before anything
+    else: C=4 with 16 rows per thread is refused (no launch, no GPU)."""
+    import ctypes
+    from opymize_amd import _hip
+    from opymize_amd.solvers.tv import TVCfg
+    lib = _hip.load()
+    c = TVCfg()
+    c.C, c.tw, c.rpt, c.steps, c.h0, c.h1 = 4, 64, 16, 0, 1.0, 1.0
+    for name in ("opy_tv_primal_f32", "opy_tv_primal_f64"):
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        rc = fn(None, None, None, None, None, None, None, None, None, ctypes.byref(c), 0, None)
+        assert rc == 1 and b"register budget" in lib.opy_last_error()
+    c.rpt = 12  # accepted by the check, then refused for the null pointers
+    rc = lib.opy_tv_primal_f32(None, None, None, None, None, None, None, None, None, ctypes.byref(c), 0, None)
+    assert rc == 1 and b"null pointer" in lib.opy_last_error()
