@@ -258,6 +258,13 @@ int opy_pdhg_residual2_f32(const opy_step_t* a, const float* z, const float* zn,
 int opy_pdhg_residual2_f64(const opy_step_t* a, const double* z, const double* zn, double b, const double* g,
                            double c1, double c2, const double* gn, int64_t n, double* work, double* out2,
                            void* stream);
+/* The same residual sums as opy_pdhg_residual2_f32 with no work argument: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under hipGraph capture, serving
+ * loops).  The 2*OPY_RED_BLOCKS partials (16 KiB) sit in a device array of the library, one per
+ * GPU, held until the process ends; every n fits, since the reduction grid is clamped to
+ * OPY_RED_BLOCKS blocks.  Calls that overlap on one GPU race on that array.  Allocates nothing. */
+int opy_pdhg_residual2_noalloc_f32(const opy_step_t* a, const float* z, const float* zn, double b, const float* g,
+                                   double c1, double c2, const float* gn, int64_t n, double* out2, void* stream);
 /* adaptive step rule on the device; pdhg.py:166-183 */
 int opy_pdhg_adapt(opy_adapt_state_t* state, const opy_adapt_params_t* params, void* stream);
 

@@ -7,7 +7,9 @@
 // the caller's work buffer (OPY_RED_BLOCKS * 3 doubles), stage 2 folds them.
 #include "opy_common.h"
 
+#include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace opy {
 namespace {
@@ -449,10 +451,59 @@ int residual2_impl(const opy_step_t* a, const T* z, const T* zn, double b, const
   return check_launch("residual");
 }
 
+// ---- residuals without a caller-supplied work buffer ---------------------
+// The partials of the two-stage residual reduction live in a file-scope
+// device array instead of a workspace argument, for callers that cannot
+// allocate (code under graph capture, serving loops).  Every device holds
+// its own copy at its own address; the host never passes the array's name
+// to a kernel, only the address hipGetSymbolAddress returns for the current
+// device.  Stage 1 writes every slot stage 2 reads, so nothing needs
+// clearing between calls; calls that overlap on one device race on it.
+constexpr int kScratchDoubles = 2 * kRedBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {nullptr};
+
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(kHipError, "residual (no-alloc): hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(kInvalid, "residual (no-alloc): device %d beyond the %d supported", dev, kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(kHipError, "residual (no-alloc): hipGetSymbolAddress on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
 }  // namespace
 }  // namespace opy
 
 using namespace opy;
+
+OPY_API int opy_pdhg_residual2_noalloc_f32(const opy_step_t* a, const float* z, const float* zn, double b,
+                                           const float* g, double c1, double c2, const float* gn, int64_t n,
+                                           double* out2, void* s) {
+  if (n < 0) return fail(kInvalid, "residual (no-alloc): negative size");
+  // the reduction grid is clamped to kRedBlocks blocks, whose partials fill the array exactly
+  const int64_t nb = std::min<int64_t>(std::max<int64_t>((n + kBlock - 1) / kBlock, 1), kRedBlocks);
+  if (2 * nb > kScratchDoubles)
+    return fail(kInvalid, "residual (no-alloc): %lld partials exceed the %d-double scratch array",
+                (long long)(2 * nb), kScratchDoubles);
+  double* work = nullptr;
+  int rc = scratch_for_current_device(&work);
+  if (rc != kOk) return rc;
+  return residual2_impl<float>(a, z, zn, b, g, c1, c2, gn, n, work, out2, s);
+}
 
 OPY_API int opy_pdhg_residual2_f32(const opy_step_t* a, const float* z, const float* zn, double b, const float* g,
                                    double c1, double c2, const float* gn, int64_t n, double* work, double* out2,
