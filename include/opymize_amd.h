@@ -17,7 +17,7 @@
  *
  * The reference's device boundary is Operator.__call__ / LinOp.__call__
  * dispatching to _call_gpu (opymize/__init__.py:64-120), which launches
- * PyCUDA kernels compiled from opymize/{linear,operators}/*.cu and the
+ * PyCUDA kernels compiled from the .cu files of opymize/linear and opymize/operators and the
  * string kernels of opymize/solvers/pdhg.py:61-79.  Each declaration below
  * names the reference code it replaces.
  */
@@ -85,7 +85,12 @@ enum {
   OPY_FN_HUBER = 12,         /* HuberPerspective            (reference epigraph.py:224-236)         */
   OPY_FN_TRUNCQUADEPI = 13,  /* TruncQuadEpiInd (2 infeas)  (reference epigraph.py:256-270)         */
   OPY_FN_EPIGRAPH_IND = 14,  /* EpigraphInd infeas          (reference epigraph.py:42-47)           */
-  OPY_FN_EPIGRAPH_SUPP = 15  /* EpigraphSupp value/infeas   (reference epigraph.py:115-126)         */
+  OPY_FN_EPIGRAPH_SUPP = 15, /* EpigraphSupp value/infeas   (reference epigraph.py:115-126)         */
+  /* no reference counterpart: the box and simplex constraints of the multi-label lifting model */
+  OPY_FN_BOX_IND = 16,       /* infeas max(0, lo - x, x - hi), lo = cs, hi = lbd                     */
+  OPY_FN_BOX_SUPP = 17,      /* its conjugate: sum max(lo z, hi z)                                   */
+  OPY_FN_SIMPLEX_IND = 18,   /* per point of width K: infeas max(|sum x - 1|, max(0, -x))            */
+  OPY_FN_SIMPLEX_SUPP = 19   /* its conjugate: sum over points of max_k z_k                          */
 };
 
 typedef struct {
@@ -239,6 +244,22 @@ int opy_epigraph_proj_f32(float* x, const uint8_t* I, const int32_t* J, const fl
                           int nfuns, int nregions, int npoints, int nsub, int maxiter, double tol, void* stream);
 int opy_epigraph_proj_f64(double* x, const uint8_t* I, const int32_t* J, const double* v, const double* b, int ndim,
                           int nfuns, int nregions, int npoints, int nsub, int maxiter, double tol, void* stream);
+
+/* ---- lifting-model projections and block preconditioners (no reference counterpart) ---------
+ * The multi-label lifting model keeps its lifted variable in a box and its per-point piece weights
+ * in the probability simplex; the reference has neither functional. */
+/* y = min(max(x, lo), hi) (x == y allowed) */
+int opy_clip_f32(const float* x, float* y, int64_t n, double lo, double hi, void* stream);
+int opy_clip_f64(const double* x, double* y, int64_t n, double lo, double hi, void* stream);
+/* in place: each of the N points' K <= 32 values onto {v >= 0, sum v = 1} (Michelot's algorithm) */
+int opy_simplex_proj_f32(float* x, int64_t N, int K, void* stream);
+int opy_simplex_proj_f64(double* x, int64_t N, int K, void* stream);
+/* in place: each block of `width` step sizes takes its smallest positive entry (0 if none).  The
+ * diagonal preconditioner of pdhg.py:189-200 gives per-element steps; a prox that is a projection onto
+ * a non-separable set (an epigraph, a simplex) is only the exact prox for a step that is constant on
+ * the set's block, and the smaller step keeps the convergence condition. */
+int opy_block_min_f32(float* x, int64_t nblocks, int width, void* stream);
+int opy_block_min_f64(double* x, int64_t nblocks, int width, void* stream);
 
 /* ---- PDHG step kernels (reference opymize/solvers/pdhg.py) ------------- */
 /* take_step: zn = z + dir*step*g; pdhg.py:62-65 */

@@ -162,6 +162,19 @@ class Functional(object):
         """Number of (sum, max, max) FP64 result triples _gpu_eval writes."""
         return 1
 
+    # Width of the blocks of entries that the prox couples (a projection onto a
+    # non-separable set: an epigraph, a simplex), or None for a prox that acts
+    # on each entry alone or has a step-independent exact form per entry.
+    precond_block = None
+
+    def precond_blocks(self):
+        """(offset, nblocks, width) ranges of x on which preconditioned steps
+        must be constant for prox(tau) to be the exact prox."""
+        w = self.precond_block
+        if not w or self.x is None:
+            return []
+        return [(0, self.x.size // w, w)]
+
     def _gpu_eval(self, x, out3):
         raise Exception("This functional does not support being called")
 

@@ -38,6 +38,10 @@ FN_HUBER = 12
 FN_TRUNCQUADEPI = 13
 FN_EPIGRAPH_IND = 14
 FN_EPIGRAPH_SUPP = 15
+FN_BOX_IND = 16
+FN_BOX_SUPP = 17
+FN_SIMPLEX_IND = 18
+FN_SIMPLEX_SUPP = 19
 
 
 class Grid(ctypes.Structure):
@@ -113,6 +117,9 @@ _SIGS = {
     "pdhg_dual_step": [_P, _P, _L, _D, _P, _P, _P, _P],
     "pdhg_residual2": [_P, _P, _P, _D, _P, _D, _D, _P, _L, _P, _P, _P],
     "fn_eval": [_I, _P, _L, _I, _P, _P, _P, _I, _P],
+    "clip": [_P, _P, _L, _D, _D, _P],
+    "simplex_proj": [_P, _L, _I, _P],
+    "block_min": [_P, _L, _I, _P],
 }
 # untyped entry points (no _f32/_f64 suffix)
 _SIGS_PLAIN = {

@@ -3,6 +3,9 @@
 // a matrix.
 #include "opy_common.h"
 
+#include <algorithm>
+#include <type_traits>
+
 namespace opy {
 namespace {
 
@@ -52,6 +55,119 @@ __global__ void __launch_bounds__(kBlock) matmul_r_kernel(const T* __restrict__ 
   y[idx] = acc;
 }
 
+// ---- FP32 per-point products on the matrix cores -------------------------
+// One wavefront forms the products of 16 points at a time as 16 x 16 tiles of
+// v_mfma_f32_16x16x4f32 (FP32 operands, FP32 accumulation), in K/4 steps.
+// Lane l of the wavefront holds the first operand's element (row l%16,
+// k = l/16), the second operand's element (k = l/16, column l%16) and the
+// result's rows 4*(l/16) + r, r = 0..3, of column l%16.
+//   point-major (MatrixMultR, PM): rows = points, columns = outputs j; the
+//     x rows are the first operand, the matrix the second;
+//   label-major (MatrixMult): rows = outputs j, columns = points; the matrix
+//     is the first operand, x the second.
+// In both, lane l reads x at point p0 + l%16 and k = 4s + l/16, and holds the
+// coefficient of x[., k = 4s + l/16] in y[., j = 16t + l%16] in register
+// m[s][t] for the whole launch.  Entries beyond K, J or N are zero operands
+// and skipped stores.
+typedef float opy_f32x4 __attribute__((ext_vector_type(4)));
+
+template <int KS, int JT, bool PM>
+__global__ void __launch_bounds__(kBlock) matmul_mfma_kernel(const float* __restrict__ x, float* __restrict__ y,
+                                                             const float* __restrict__ A, int64_t N, int J, int K,
+                                                             int trans, int add) {
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  float m[KS][JT];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int k = 4 * s + lq, j = 16 * t + lr;
+      float c = 0.0f;
+      if (k < K && j < J) {
+        // PM: y[i, j] = sum_k x[i, k] A(k, j), A(k, j) = A[k*J + j] (A is K x J) or A[j*K + k] (trans)
+        // label-major: y[j, i] = sum_k A(j, k) x[k, i], A(j, k) = A[j*K + k] (A is J x K) or A[k*J + j] (trans)
+        const bool kj = PM ? !trans : (bool)trans;
+        c = kj ? A[k * J + j] : A[j * K + k];
+      }
+      m[s][t] = c;
+    }
+  }
+  const int64_t ntiles = (N + 15) >> 4;
+  const int64_t waves_per_block = blockDim.x >> 6;
+  const int64_t nwaves = (int64_t)gridDim.x * waves_per_block;
+  for (int64_t tile = (int64_t)blockIdx.x * waves_per_block + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
+    const int64_t p0 = tile << 4;
+    const int64_t px = p0 + lr;
+    float xv[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int k = 4 * s + lq;
+      xv[s] = 0.0f;
+      if (px < N && k < K) xv[s] = PM ? x[px * K + k] : x[(int64_t)k * N + px];
+    }
+    opy_f32x4 acc[JT];
+#pragma unroll
+    for (int t = 0; t < JT; ++t) acc[t] = opy_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+#pragma unroll
+      for (int t = 0; t < JT; ++t) {
+        acc[t] = PM ? __builtin_amdgcn_mfma_f32_16x16x4f32(xv[s], m[s][t], acc[t], 0, 0, 0)
+                    : __builtin_amdgcn_mfma_f32_16x16x4f32(m[s][t], xv[s], acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int64_t p, idx;
+        int j;
+        if (PM) {
+          p = p0 + 4 * lq + r;
+          j = 16 * t + lr;
+          idx = p * J + j;
+        } else {
+          p = px;
+          j = 16 * t + 4 * lq + r;
+          idx = (int64_t)j * N + p;
+        }
+        if (p < N && j < J) y[idx] = add ? y[idx] + acc[t][r] : acc[t][r];
+      }
+    }
+  }
+}
+
+// the matrix-core path covers K, J <= 64 in FP32; returns false if the shape is outside it
+template <bool PM>
+bool launch_matmul_mfma(const float* x, float* y, const float* A, int64_t N, int J, int K, int trans, int add,
+                        hipStream_t st) {
+  if (K > 64 || J > 64) return false;
+  const int64_t ntiles = (N + 15) / 16;
+  const int64_t blocks = std::min<int64_t>((ntiles + 3) / 4, 1 << 16);
+  const dim3 g((unsigned)std::max<int64_t>(blocks, 1)), b(kBlock);
+  const int ks = K <= 4 ? 1 : K <= 8 ? 2 : K <= 16 ? 4 : K <= 32 ? 8 : 16;
+  const int jt = J <= 16 ? 1 : J <= 32 ? 2 : 4;
+#define OPY_MM(KS_, JT_) \
+  hipLaunchKernelGGL((matmul_mfma_kernel<KS_, JT_, PM>), g, b, 0, st, x, y, A, N, J, K, trans, add)
+#define OPY_MM_J(KS_)                 \
+  switch (jt) {                       \
+    case 1: OPY_MM(KS_, 1); break;    \
+    case 2: OPY_MM(KS_, 2); break;    \
+    default: OPY_MM(KS_, 4); break;   \
+  }
+  switch (ks) {
+    case 1: OPY_MM_J(1); break;
+    case 2: OPY_MM_J(2); break;
+    case 4: OPY_MM_J(4); break;
+    case 8: OPY_MM_J(8); break;
+    default: OPY_MM_J(16); break;
+  }
+#undef OPY_MM_J
+#undef OPY_MM
+  return true;
+}
+
 // y (+)= A x for a CSR matrix; one thread per row.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) csr_spmv_kernel(int64_t nrows, const int64_t* __restrict__ indptr,
@@ -82,9 +198,13 @@ int matmul_impl(const T* x, T* y, const T* A, int64_t N, int J, int K, int trans
   if (N < 0 || J < 1 || K < 1) return fail(kInvalid, "matmul: bad shape");
   if (N > 0 && (!x || !y || !A)) return fail(kInvalid, "matmul: null pointer");
   const int64_t n = N * J;
-  if (n > 0)
+  if (n > 0) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (launch_matmul_mfma<false>(x, y, A, N, J, K, trans, add, as_stream(s))) return check_launch("matmul");
+    }
     hipLaunchKernelGGL((matmul_kernel<T>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), x, y, A, N, J, K, trans,
                        add);
+  }
   return check_launch("matmul");
 }
 
@@ -95,6 +215,9 @@ int matmul_r_impl(const T* x, T* y, const T* A, int64_t N, int J, int K, int tra
   const int64_t n = N * J;
   const size_t abytes = (size_t)J * K * sizeof(T);
   const int use_lds = abytes <= 32 * 1024;
+  if constexpr (std::is_same<T, float>::value) {
+    if (n > 0 && launch_matmul_mfma<true>(x, y, A, N, J, K, trans, add, as_stream(s))) return check_launch("matmul_r");
+  }
   if (n > 0)
     hipLaunchKernelGGL((matmul_r_kernel<T>), dim3(grid_for(n)), dim3(kBlock), use_lds ? abytes : 0, as_stream(s), x,
                        y, A, N, J, K, trans, add, use_lds);

@@ -373,6 +373,40 @@ struct FnOp {
         }
         return;
       }
+      case OPY_FN_BOX_IND: {  // max(0, lo - x, x - hi) with lo = p.cs, hi = p.lbd
+        double inf = 0.0;
+        for (int k = 0; k < width; ++k) {
+          const double v = (double)xi[k];
+          inf = fmax(inf, fmax(p.cs - v, v - p.lbd));
+        }
+        m1 = fmax(m1, inf);
+        return;
+      }
+      case OPY_FN_BOX_SUPP: {  // support function of [lo, hi]^n: sum max(lo z, hi z)
+        double v = 0.0;
+        for (int k = 0; k < width; ++k) {
+          const double z = (double)xi[k];
+          v += fmax(p.cs * z, p.lbd * z);
+        }
+        s += v;
+        return;
+      }
+      case OPY_FN_SIMPLEX_IND: {  // per point: max(|sum x - 1|, max(0, -x))
+        double sum = 0.0, inf = 0.0;
+        for (int k = 0; k < width; ++k) {
+          const double v = (double)xi[k];
+          sum += v;
+          inf = fmax(inf, -v);
+        }
+        m1 = fmax(m1, fmax(inf, fabs(sum - 1.0)));
+        return;
+      }
+      case OPY_FN_SIMPLEX_SUPP: {  // support function of the simplex per point: max_k z_k
+        double mx = (double)xi[0];
+        for (int k = 1; k < width; ++k) mx = fmax(mx, (double)xi[k]);
+        s += mx;
+        return;
+      }
       default:
         return;
     }
@@ -407,6 +441,12 @@ int fn_eval_impl(int kind, const T* x, int64_t npts, int width, const opy_fn_par
     case OPY_FN_QUADEPI_SUPP:
     case OPY_FN_HUBER:
     case OPY_FN_TRUNCQUADEPI:
+    case OPY_FN_SIMPLEX_IND:
+    case OPY_FN_SIMPLEX_SUPP:
+      break;
+    case OPY_FN_BOX_IND:
+    case OPY_FN_BOX_SUPP:
+      if (!(prm->cs <= prm->lbd)) return fail(kInvalid, "fn_eval: box needs lo <= hi");
       break;
     case OPY_FN_EPIGRAPH_IND:
       if (!prm->ei || !prm->ej || !prm->ev || !prm->eb || prm->nfuns < 1 || prm->npoints < 1 || prm->nsub < 1)

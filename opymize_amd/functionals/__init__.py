@@ -5,7 +5,8 @@ import torch
 from opymize_amd import Functional, Variable
 from opymize_amd.operators import SplitOp
 from opymize_amd.functionals.affine import AffineFct, ConstFct, ZeroFct, MaskedAffineFct
-from opymize_amd.functionals.indicators import IndicatorFct, ConstrainFct, PositivityFct, NegativityFct
+from opymize_amd.functionals.indicators import (IndicatorFct, ConstrainFct, PositivityFct, NegativityFct, BoxFct,
+                                                BoxSupp, SimplexFct, SimplexSupp)
 from opymize_amd.functionals.l1norms import L1Norms, L1NormsConj
 from opymize_amd.functionals.quad import SSD, L1Data, L1DataConj
 from opymize_amd.functionals.epigraph import (QuadEpiInd, QuadEpiSupp, QuadSupport, HuberPerspective,
@@ -25,6 +26,13 @@ class SplitSum(Functional):
 
     def nslots(self):
         return sum(F.nslots() for F in self.fcts)
+
+    def precond_blocks(self):
+        out, off = [], 0
+        for F in self.fcts:
+            out += [(off + o, nb, w) for o, nb, w in F.precond_blocks()]
+            off += F.x.size
+        return out
 
     def _gpu_eval(self, x, out):
         o = 0
@@ -53,6 +61,6 @@ class SplitSum(Functional):
 
 
 __all__ = ["SplitSum", "AffineFct", "ConstFct", "ZeroFct", "MaskedAffineFct", "IndicatorFct", "ConstrainFct",
-           "PositivityFct", "NegativityFct", "L1Norms", "L1NormsConj", "SSD", "L1Data", "L1DataConj",
+           "BoxFct", "BoxSupp", "SimplexFct", "SimplexSupp", "PositivityFct", "NegativityFct", "L1Norms", "L1NormsConj", "SSD", "L1Data", "L1DataConj",
            "QuadEpiInd", "QuadEpiSupp", "QuadSupport", "HuberPerspective", "TruncQuadEpiInd",
            "quad_dual_coefficients"]

@@ -48,6 +48,7 @@ class EpigraphInd(Functional):
         self.b = self.conj.b  # right-hand side of the constraint system, as in the reference
         self._fb = b
         self._prox = EpigraphProj(I, J, v, b, Ab=(self.A, self.b))
+        self.precond_block = self.v.shape[1] + 1  # one QP per (region, function) block
         self._const = DeviceConstants()
 
     def _gpu_eval(self, x, out):
@@ -178,6 +179,7 @@ class QuadEpiInd(_QuadBase):
         else:
             self.conj = conj
         self._prox = QuadEpiProj(N, M, alph=self.a, b=b, c=c)
+        self.precond_block = self.M + 1  # the projection couples each point's M+1 entries
 
     def _gpu_eval(self, x, out):
         fn_eval(_hip.FN_QUADEPI_IND, x, self.M + 1, out, self._params())
@@ -235,6 +237,7 @@ class TruncQuadEpiInd(Functional):
         else:
             self.conj = conj
         self._prox = QuadEpiProj(N, M, lbd=self.lbd, alph=self.alph)
+        self.precond_block = self.M + 1
 
     def _gpu_eval(self, x, out):
         prm = _hip.FnParams()

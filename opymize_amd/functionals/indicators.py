@@ -5,7 +5,7 @@ from opymize_amd import Functional, Variable, _hip
 from opymize_amd._devcache import DeviceConstants
 from opymize_amd.functionals._base import fn_eval, vptr
 from opymize_amd.operators.affine import ConstOp, ConstrainOp
-from opymize_amd.operators.proj import PosProj, NegProj
+from opymize_amd.operators.proj import PosProj, NegProj, ClipOp, SimplexProj
 
 
 class IndicatorFct(Functional):
@@ -112,3 +112,93 @@ class NegativityFct(Functional):
 
     def prox(self, tau):
         return self._prox
+
+
+class BoxFct(Functional):
+    """F(x) = 0 if lo <= x <= hi entrywise, else inf (no reference
+    counterpart: the range of the lifted variable of the multi-label lifting
+    model).  Its conjugate is the support function BoxSupp, finite everywhere,
+    so a dual objective with a box-constrained primal carries no
+    infeasibility; the prox is ClipOp."""
+    device_step = True  # prox is independent of the step
+
+    def __init__(self, N, lo=0.0, hi=1.0, conj=None):
+        Functional.__init__(self)
+        if not lo <= hi:
+            raise ValueError("BoxFct needs lo <= hi")
+        self.x = Variable(N)
+        self.lo, self.hi = float(lo), float(hi)
+        self.conj = BoxSupp(N, lo, hi, conj=self) if conj is None else conj
+        self._prox = ClipOp(N, self.lo, self.hi)
+
+    def _gpu_eval(self, x, out):
+        prm = _hip.FnParams()
+        prm.cs, prm.lbd = self.lo, self.hi
+        fn_eval(_hip.FN_BOX_IND, x, 1, out, prm)
+
+    def _gpu_finish(self, v):
+        return 0.0, float(v[1])
+
+    def prox(self, tau):
+        return self._prox
+
+
+class BoxSupp(Functional):
+    """F(z) = sum_i max(lo z_i, hi z_i), the support function of [lo, hi]^N
+    (conjugate of BoxFct)."""
+
+    def __init__(self, N, lo=0.0, hi=1.0, conj=None):
+        Functional.__init__(self)
+        self.x = Variable(N)
+        self.lo, self.hi = float(lo), float(hi)
+        self.conj = BoxFct(N, lo, hi, conj=self) if conj is None else conj
+
+    def _gpu_eval(self, x, out):
+        prm = _hip.FnParams()
+        prm.cs, prm.lbd = self.lo, self.hi
+        fn_eval(_hip.FN_BOX_SUPP, x, 1, out, prm)
+
+    def _gpu_finish(self, v):
+        return float(v[0]), 0.0
+
+
+class SimplexFct(Functional):
+    """F(x) = 0 if every point's K values lie in the probability simplex
+    {v >= 0, sum v = 1}, else inf; infeasibility max(|sum v - 1|, max(0, -v))
+    (no reference counterpart: the piece weights of the multi-label lifting
+    model).  Conjugate SimplexSupp (finite everywhere); prox SimplexProj."""
+    device_step = True  # prox is independent of the step
+
+    def __init__(self, N, K, conj=None):
+        Functional.__init__(self)
+        self.N, self.K = int(N), int(K)
+        self.x = Variable((self.N, self.K))
+        self.conj = SimplexSupp(N, K, conj=self) if conj is None else conj
+        self._prox = SimplexProj(N, K)
+        self.precond_block = self.K  # the projection couples each point's K entries
+
+    def _gpu_eval(self, x, out):
+        fn_eval(_hip.FN_SIMPLEX_IND, x, self.K, out)
+
+    def _gpu_finish(self, v):
+        return 0.0, float(v[1])
+
+    def prox(self, tau):
+        return self._prox
+
+
+class SimplexSupp(Functional):
+    """F(z) = sum over points of max_k z_k, the support function of the
+    simplex per point (conjugate of SimplexFct)."""
+
+    def __init__(self, N, K, conj=None):
+        Functional.__init__(self)
+        self.N, self.K = int(N), int(K)
+        self.x = Variable((self.N, self.K))
+        self.conj = SimplexFct(N, K, conj=self) if conj is None else conj
+
+    def _gpu_eval(self, x, out):
+        fn_eval(_hip.FN_SIMPLEX_SUPP, x, self.K, out)
+
+    def _gpu_finish(self, v):
+        return float(v[0]), 0.0

@@ -1,0 +1,107 @@
+"""The multi-label lifting model: the data cost rho(s) = (s - f)^2 and the
+regularizer lam*TV(s) of a 2-D image, lifted to L labels with the
+sublabel-accurate relaxation of Moellenhoff, Laude, Moeller, Lellmann &
+Cremers (CVPR 2016), assembled from the package's drop-in operators and
+functionals and solved by PDHG(G, F, A).solve(steps="precond").
+
+The reference has no such model, only its parts: GradientOp
+(linear/diff.py:67-144), MatrixMultR (linear/einsum.py:58-97), L1Norms
+(functionals/l1norms.py:8-92) and QuadEpiSupp / QuadEpiInd with their
+projection QuadEpiProj (functionals/epigraph.py:129-201,
+operators/proj.py:278-374).
+
+Labels gam_1 < ... < gam_L are equally spaced by h; k = L - 1 pieces.  The
+lifted point of a value s = gam_i + alpha*h (piece i) is
+U = (1, ..., 1, alpha, 0, ..., 0) in [0, 1]^k, and s = gam_1 + h*sum_i U_i.
+Primal x = (U, r), U (N, k) in [0, 1], r (N, k) per pixel in the probability
+simplex; dual y = (p, z):
+
+  p (N*k, 3)  = grad U over the (H, W, k) grid whose label axis has infinite
+                spacing, i.e. (dU_i/dx, dU_i/dy, 0) per (pixel, piece), with
+                F_p = lam*h * sum |p| (its conjugate projects on balls of
+                radius lam*h);
+  z (N*k, 2)  = (a_i, -r_i) per (pixel, piece),
+                a_i = h U_i + gam_i r_i - h sum_{l>i} r_l, two per-pixel
+                MatrixMultR products U A_U + r A_r of shape k x 2k; with
+                F_z = QuadEpiSupp(a=2, b=-2f, c=f^2), the perspective
+                r_i rho(a_i/r_i) of the data cost, whose conjugate projects
+                onto epi{nu^2/4 + f nu} (the quadratic-epigraph projection
+                with the per-pixel shift (-2f, -f^2)).
+
+For convex rho the lifted energy bounds sum (s - f)^2 + lam*TV(s) of the
+unlifted s from above and agrees with it at lifted points (exactly with one
+piece), so the solution's s is the ROF solution up to the relaxation gap of
+the discrete regularizer at jumps across several pieces.
+"""
+import numpy as np
+
+from opymize_amd.functionals import BoxFct, SimplexFct, L1Norms, QuadEpiSupp, SplitSum
+from opymize_amd.linear import BlockOp, GradientOp, MatrixMultR
+
+
+def labels(L, lo, hi):
+    """L equally spaced labels on [lo, hi]."""
+    if int(L) < 2 or not lo < hi:
+        raise ValueError("need L >= 2 labels on an interval lo < hi")
+    return np.linspace(float(lo), float(hi), int(L))
+
+
+def pair_matrices(gam):
+    """(A_U, A_r), both k x 2k, of the pairs z = U A_U + r A_r per pixel
+    (entry (l, 2i) of A_r is the coefficient of r_l in a_i, (i, 2i+1) the -1
+    of the pair's second entry)."""
+    gam = np.asarray(gam, dtype=np.float64)
+    k = gam.size - 1
+    h = gam[1] - gam[0]
+    AU = np.zeros((k, 2 * k))
+    Ar = np.zeros((k, 2 * k))
+    idx = np.arange(k)
+    AU[idx, 2 * idx] = h
+    Ar[idx, 2 * idx] = gam[:k]
+    Ar[idx, 2 * idx + 1] = -1.0
+    upper = np.triu(np.ones((k, k)), 1).T  # [l, i] = 1 where l > i
+    Ar[:, 0::2] -= h * upper
+    return AU, Ar
+
+
+def sublabel_tv(f, L, lam, lo=None, hi=None):
+    """(G, F, A, gam) of the lifted problem for the image f (H, W): minimise
+    G(x) + F(Ax) with PDHG(G, F, A).solve(steps="precond")."""
+    f = np.asarray(f, dtype=np.float64)
+    if f.ndim != 2:
+        raise ValueError("sublabel_tv lifts a 2-D grayscale image, got shape %s" % (f.shape,))
+    H, W = f.shape
+    N = H * W
+    gam = labels(L, f.min() if lo is None else lo, f.max() if hi is None else hi)
+    k = gam.size - 1
+    h = gam[1] - gam[0]
+    AU, Ar = pair_matrices(gam)
+    grad = GradientOp((H, W, k), 1, scheme="forward", imageh=(1.0, 1.0, np.inf))
+    A = BlockOp([[grad, 0], [MatrixMultR(N, AU), MatrixMultR(N, Ar)]])
+    fp = np.repeat(f.ravel(), k)
+    G = SplitSum([BoxFct(N * k, 0.0, 1.0), SimplexFct(N, k)])
+    F = SplitSum([L1Norms(N * k, (3, 1), lam * h), QuadEpiSupp(N * k, 1, a=2.0, b=-2.0 * fp[:, None], c=fp ** 2)])
+    return G, F, A, gam
+
+
+def lift(s, gam):
+    """(U, r) of the lifted point of s (N,), values clipped to [gam_1, gam_L]."""
+    s = np.asarray(s, dtype=np.float64).ravel()
+    gam = np.asarray(gam, dtype=np.float64)
+    k = gam.size - 1
+    t = np.clip((s - gam[0]) / (gam[1] - gam[0]), 0.0, float(k))
+    i = np.minimum(np.floor(t).astype(np.int64), k - 1)
+    U = (np.arange(k)[None, :] < i[:, None]).astype(np.float64)
+    U[np.arange(s.size), i] = t - i
+    r = np.zeros((s.size, k))
+    r[np.arange(s.size), i] = 1.0
+    return U, r
+
+
+def unlift(x, gam, N):
+    """s = gam_1 + h * sum_i U_i for the N pixels, from the primal x = (U, r)
+    (or U alone: only the first N*k entries are read)."""
+    gam = np.asarray(gam, dtype=np.float64)
+    k = gam.size - 1
+    U = np.asarray(x, dtype=np.float64).ravel()[:N * k].reshape(N, k)
+    return gam[0] + (gam[1] - gam[0]) * U.sum(axis=1)

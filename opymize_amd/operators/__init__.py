@@ -2,7 +2,8 @@
 from opymize_amd import Operator, Variable
 from opymize_amd.operators.affine import (ConstOp, ConstrainOp, ShiftScaleOp, SSDProxOp, L1DataProxOp,
                                           StepArg)
-from opymize_amd.operators.proj import PosProj, NegProj, L1NormsProj, QuadEpiProj, EpigraphProj, epigraph_Ab
+from opymize_amd.operators.proj import (PosProj, NegProj, L1NormsProj, QuadEpiProj, EpigraphProj, epigraph_Ab,
+                                       ClipOp, SimplexProj)
 
 
 class SplitOp(Operator):
@@ -29,5 +30,5 @@ class SplitOp(Operator):
             return BlockOp(results)
 
 
-__all__ = ["SplitOp", "ConstOp", "ConstrainOp", "ShiftScaleOp", "SSDProxOp", "L1DataProxOp", "StepArg",
+__all__ = ["ClipOp", "SimplexProj", "SplitOp", "ConstOp", "ConstrainOp", "ShiftScaleOp", "SSDProxOp", "L1DataProxOp", "StepArg",
            "PosProj", "NegProj", "L1NormsProj", "QuadEpiProj"]

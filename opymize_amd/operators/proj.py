@@ -191,3 +191,46 @@ class EpigraphProj(Operator):
         db = self._const.get("b", self.b, t.dtype)
         _hip.call("epigraph_proj", t.dtype, _hip.ptr(t), _hip.ptr(dI), _hip.ptr(dJ), _hip.ptr(dv), _hip.ptr(db),
                   self.v.shape[1], nfuns, nregions, npoints, nsub, self.maxiter, self.tol, _hip.stream())
+
+
+class ClipOp(Operator):
+    """T(x) = min(max(x, lo), hi): projection onto the box [lo, hi]^N (no
+    reference counterpart; the prox of BoxFct)."""
+
+    def __init__(self, N, lo=0.0, hi=1.0):
+        Operator.__init__(self)
+        if not lo <= hi:
+            raise ValueError("ClipOp needs lo <= hi, got %g > %g" % (lo, hi))
+        self.x = Variable(N)
+        self.y = self.x
+        self.lo, self.hi = float(lo), float(hi)
+
+    def _call_gpu(self, x, y=None, add=False, jacobian=False):
+        assert not add
+        assert not jacobian
+        y = x if y is None else y
+        if x.numel() != self.x.size or y.numel() != self.x.size:
+            raise ValueError("ClipOp: %d -> %d entries, expected %d" % (x.numel(), y.numel(), self.x.size))
+        _hip.call("clip", x.dtype, _hip.ptr(x), _hip.ptr(y), x.numel(), self.lo, self.hi, _hip.stream())
+
+
+class SimplexProj(Operator):
+    """T(x)[i] = projection of the i-th point's K values onto the probability
+    simplex {v >= 0, sum v = 1} (no reference counterpart; the prox of
+    SimplexFct).  Michelot's finite algorithm, one thread per point."""
+
+    def __init__(self, N, K):
+        Operator.__init__(self)
+        if not 1 <= K <= 32:
+            raise ValueError("SimplexProj supports 1 <= K <= 32, got %d" % K)
+        self.N, self.K = int(N), int(K)
+        self.x = Variable((self.N, self.K))
+        self.y = self.x
+
+    def _call_gpu(self, x, y=None, add=False, jacobian=False):
+        assert not add
+        assert not jacobian
+        if x.numel() != self.x.size:
+            raise ValueError("SimplexProj: %d entries, expected %d" % (x.numel(), self.x.size))
+        t = _inplace_target(x, y)
+        _hip.call("simplex_proj", t.dtype, _hip.ptr(t), self.N, self.K, _hip.stream())

@@ -155,6 +155,13 @@ class PDHG(object):
             eps = float(np.spacing(1))
             _hip.call("recip_gt", dtype, _hip.ptr(ysigma), ysigma.numel(), eps, _hip.stream())
             _hip.call("recip_gt", dtype, _hip.ptr(xtau), xtau.numel(), eps, _hip.stream())
+            # A prox that projects onto a set coupling a block of entries (an epigraph, a
+            # simplex) is the exact prox only for a step that is constant on the block: the
+            # reference's per-element steps give such a PDHG a fixed point off the solution.
+            # Each block takes its smallest positive step, which keeps the convergence bound.
+            for vec, fct in ((ysigma, self.f.conj), (xtau, self.g)):
+                for off, nb, w in fct.precond_blocks():
+                    _hip.call("block_min", dtype, _hip.addr(vec, off), nb, w, _hip.stream())
             d["xtau"], d["ysigma"] = xtau, ysigma
             self.gprox = self.g.prox(xtau)
             self.fconjprox = self.f.conj.prox(ysigma)

@@ -10,12 +10,30 @@ class Problem(object):
     """min_x G(x) + F(Ax) given by its sparse A, the two prox maps and the
     four (value, infeasibility) evaluations used by the gap."""
 
-    def __init__(self, A, gprox, fconjprox, G, F, Gc, Fc, gamma=0.0):
+    def __init__(self, A, gprox, fconjprox, G, F, Gc, Fc, gamma=0.0, xblocks=(), yblocks=()):
         self.A = A.tocsr()
         self.AT = self.A.T.tocsr()
         self.gprox, self.fconjprox = gprox, fconjprox
         self.G, self.F, self.Gc, self.Fc = G, F, Gc, Fc
         self.gamma = gamma  # strong convexity modulus of G (accelerated steps)
+        # (offset, nblocks, width): ranges of x / y whose prox is a projection onto a set that
+        # couples each block of `width` entries; preconditioned steps are made constant there
+        self.xblocks, self.yblocks = list(xblocks), list(yblocks)
+
+
+def block_min(v, blocks):
+    """Every block of `width` entries in the ranges takes its smallest positive
+    entry (0 if none).  The reference's diagonal preconditioner (pdhg.py:189-200)
+    gives per-element steps, but a projection onto a set that couples a block
+    is the prox only for a step that is constant on the block; the smallest
+    step keeps the convergence condition |Sigma^1/2 K T^1/2| <= 1."""
+    v = np.array(v, dtype=np.float64)
+    for off, nb, w in blocks:
+        b = v[off:off + nb * w].reshape(nb, w)
+        m = np.where(b > 0, b, np.inf).min(axis=1)
+        m[~np.isfinite(m)] = 0.0
+        v[off:off + nb * w] = np.repeat(m, w)
+    return v
 
 
 def tv_problem(f, dims, lam, data="l2", tv_lbd=1.0, matrixnorm="frobenius", h=None, w=None):
@@ -127,6 +145,8 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
         eps = np.spacing(1)
         ysig[ysig > eps] = 1.0 / ysig[ysig > eps]
         xtau[xtau > eps] = 1.0 / xtau[xtau > eps]
+        ysig = block_min(ysig, prob.yblocks)
+        xtau = block_min(xtau, prob.xblocks)
         tau, sigma = xtau, ysig
     else:
         bnd = step_bound

@@ -81,8 +81,12 @@ def test_laplacian(gpu, dtype, bc):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("shape,N,trans", [((5, 7), 33, False), ((5, 7), 33, True), ((16, 16), 1000, False),
-                                           ((30, 15), 257, True)])
+                                           ((30, 15), 257, True), ((15, 30), 4099, False), ((15, 30), 4099, True),
+                                           ((64, 64), 70, False), ((1, 1), 17, True), ((16, 33), 100, True),
+                                           ((65, 3), 50, False), ((3, 65), 50, True)])
 def test_matrix_mult(gpu, dtype, shape, N, trans):
+    """Both layouts, both orientations; in FP32, K, J <= 64 run on the matrix
+    cores (the 65-wide shapes take the scalar kernel)."""
     from opymize_amd.linear import MatrixMult, MatrixMultR
     rng = np.random.default_rng(2)
     A = rng.standard_normal(shape)
