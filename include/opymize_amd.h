@@ -431,6 +431,54 @@ int opy_tv_reset_f64(const opy_tv_inst_t* insts, const int32_t* flags, int B, in
 int opy_tv_compact(const opy_tv_inst_t* insts, const opy_tv_check_t* checks, int B, int32_t* active_inst,
                    int32_t* act_off, int32_t* nact, int64_t* iter_counter, int iters_in_block, void* stream);
 
+/* ---- fused, batched multi-label lifting --------------------------------
+ * PDHG with preconditioned steps on the sublabel-accurate TV lifting of many images of uneven size
+ * and label count (opymize_amd/lifting.py): the generic chain of PDHG.solve(steps="precond") on that
+ * model (reference pdhg.py:124-153 over GradientOp, MatrixMultR, L1NormsProj, QuadEpiProj and the
+ * box / simplex projections) as two passes per iteration.  Instance b keeps its k = L-1 pieces as
+ * planes of H*W values: U and r at offk + i*H*W, p and z at 2*offk + (2i + c)*H*W (c = 0 the row
+ * difference / nu, c = 1 the column difference / s), f at off.  Tiles are 64 x 4 pixels. */
+typedef struct {
+  int64_t off;    /* pixel offset of the instance's f */
+  int64_t offk;   /* plane offset of U and r (p and z at twice it) */
+  int32_t H, W;
+  int32_t K;      /* pieces (labels - 1), 1..32 */
+  int32_t tile0;  /* first per-tile partial slot */
+  int32_t ntiles; /* ceil(H/4) * ceil(W/64) */
+  int32_t pad;
+  double lam;     /* TV weight */
+  double g0, h;   /* first label, label spacing */
+  double tau_r;   /* primal step of the piece weights: min over l of 1/(|g_l| + h l + 1) */
+} opy_lift_inst_t;
+/* primal pass, in place: U <- clip(U - tau K^T y), r <- simplex(r - tau K^T y); Ub, rb <- 2 x+ - x.
+ * kmax = the largest K of the batch; `grid` blocks walk the running tiles. */
+int opy_lift_primal_f32(float* U, float* r, float* Ub, float* rb, const float* p, const float* z,
+                        const opy_lift_inst_t* insts, const int32_t* active_inst, const int32_t* act_off,
+                        const int32_t* nact, int kmax, int grid, void* stream);
+int opy_lift_primal_f64(double* U, double* r, double* Ub, double* rb, const double* p, const double* z,
+                        const opy_lift_inst_t* insts, const int32_t* active_inst, const int32_t* act_off,
+                        const int32_t* nact, int kmax, int grid, void* stream);
+/* dual pass, in place: p <- ball projection, z <- epigraph projection of y + sigma K x_bar */
+int opy_lift_dual_f32(const float* Ub, const float* rb, float* p, float* z, const float* f,
+                      const opy_lift_inst_t* insts, const int32_t* active_inst, const int32_t* act_off,
+                      const int32_t* nact, int kmax, int grid, void* stream);
+int opy_lift_dual_f64(const double* Ub, const double* rb, double* p, double* z, const double* f,
+                      const opy_lift_inst_t* insts, const int32_t* active_inst, const int32_t* act_off,
+                      const int32_t* nact, int kmax, int grid, void* stream);
+/* FP64 gap partials, 8 per tile (reference pdhg.py:23-44 for this model) */
+int opy_lift_gap_f32(const float* U, const float* r, const float* p, const float* z, const float* f,
+                     const opy_lift_inst_t* insts, const int32_t* active_inst, const int32_t* act_off,
+                     const int32_t* nact, int kmax, double* partials, int grid, void* stream);
+int opy_lift_gap_f64(const double* U, const double* r, const double* p, const double* z, const double* f,
+                     const opy_lift_inst_t* insts, const int32_t* active_inst, const int32_t* act_off,
+                     const int32_t* nact, int kmax, double* partials, int grid, void* stream);
+/* per-instance objectives, gap and termination (reference pdhg.py:299-305); B blocks */
+int opy_lift_check(const opy_lift_inst_t* insts, opy_tv_check_t* checks, const double* partials, int B,
+                   double tol_gap, double tol_infeas, const int64_t* iter_counter, int iters_in_block, void* stream);
+/* running-instance list and iteration counter, as opy_tv_compact */
+int opy_lift_compact(const opy_lift_inst_t* insts, const opy_tv_check_t* checks, int B, int32_t* active_inst,
+                     int32_t* act_off, int32_t* nact, int64_t* iter_counter, int iters_in_block, void* stream);
+
 /* ---- warm-start network: 3x3 convolutions, 32 -> 32 channels, NHWC float32 ----
  * No counterpart in the reference (its solvers always start from zero or a
  * user-supplied continue_at, pdhg.py:236-241); these kernels train and run the

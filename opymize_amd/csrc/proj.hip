@@ -4,6 +4,7 @@
 //  - QuadEpiProj: projection onto the epigraph of a (shifted, optionally
 //    truncated) paraboloid (reference proj.py:278-374, proj.cu:110-219)
 #include "opy_common.h"
+#include "quadepi.h"
 
 #include <cmath>
 
@@ -102,25 +103,6 @@ __global__ void __launch_bounds__(kBlock) l1proj_spec_kernel(T* __restrict__ x, 
     xi[m * step_m] = (T)(a0 * m11 + a1 * m21);
     xi[m * step_m + step_j] = (T)(a0 * m12 + a1 * m22);
   }
-}
-
-// Real root of y^3 + a y + b = 0 by Cardano's formula (one real root) or the
-// trigonometric form (three real roots; the largest is returned).
-template <typename T>
-__device__ inline T reduced_monic_cubic(T a, T b) {
-  if (a == T(0)) return b == T(0) ? T(0) : cbrt(-b);
-  if (b == T(0) && a > T(0)) return T(0);
-  const T Q = a / T(3);
-  const T R = -b / T(2);
-  const T Q3 = Q * Q * Q;
-  const T D = Q3 + R * R;
-  if (D <= T(0)) {
-    const T th = acos(R / sqrt(-Q3));
-    return T(2) * sqrt(-Q) * cos(th / T(3));
-  }
-  T A = cbrt(fabs(R) + sqrt(D));
-  if (R < T(0)) A = -A;
-  return A - Q / A;
 }
 
 // x_i = (x1 in R^M, x2) projected onto {x2 >= 0.5*alph*|x1 - s1|^2 + s2},

@@ -81,6 +81,8 @@ def sublabel_tv(f, L, lam, lo=None, hi=None):
     fp = np.repeat(f.ravel(), k)
     G = SplitSum([BoxFct(N * k, 0.0, 1.0), SimplexFct(N, k)])
     F = SplitSum([L1Norms(N * k, (3, 1), lam * h), QuadEpiSupp(N * k, 1, a=2.0, b=-2.0 * fp[:, None], c=fp ** 2)])
+    # lets PDHG.solve(steps="precond") run the fused kernels of this model (solvers/lift.py)
+    A.lifting_model = {"f": f, "L": int(L), "lam": float(lam), "lo": float(gam[0]), "hi": float(gam[-1])}
     return G, F, A, gam
 
 

@@ -300,6 +300,10 @@ class PDHG(object):
         if not use_gpu:
             raise ValueError("opymize_amd implements the GPU path of PDHG (use_gpu=True)")
         _hip.require_device()
+        lift = getattr(self.linop, "lifting_model", None)
+        if fused and lift is not None and steps == "precond" and term_pd_res is None and term_maxiter > 0 \
+                and continue_at is None:
+            return self._solve_fused_lift(lift, precision, term_pd_gap, term_maxiter, granularity, cbfun, use_graph)
         if fused and term_pd_res is None and term_maxiter > 0:
             model = fused_tv_model(self.g, self.f, self.linop)
             if model is not None:
@@ -466,6 +470,35 @@ class PDHG(object):
             eng.solve(steps=steps_mode, step_bound=bnd if bnd is not None else 1.0, step_factor=step_factor,
                       tol=term_pd_gap[0], tol_infeas=term_pd_gap[1], max_iters=term_maxiter, block_iters=block,
                       check_every=every, certify=False, use_graph=use_graph, callback=cb)
+        torch.cuda.synchronize()
+        return self.info
+
+    def _solve_fused_lift(self, model, precision, term_pd_gap, term_maxiter, granularity, cbfun, use_graph):
+        """The multi-label lifting model of opymize_amd.lifting through the fused
+        kernels (same iterates as the generic chain with steps="precond")."""
+        from opymize_amd.solvers.lift import LiftBatch
+        if not isinstance(term_pd_gap, tuple):
+            term_pd_gap = (term_pd_gap, term_pd_gap)
+        eng = LiftBatch([model["f"]], model["L"], model["lam"], lo=model["lo"], hi=model["hi"],
+                        precision=precision)
+        block = 100 if granularity % 100 == 0 else granularity
+        self.constvars = {"theta": 1.0, "precond": True, "fused": True}
+        logging.info("Fused lifting kernels (%d labels, preconditioned steps)" % model["L"])
+        self.log_header(False)
+        self._fused = eng
+        with GracefulInterruptHandler() as interrupt_hdl:
+            def cb(e, it):
+                r = e.results()[0]
+                self.info.update({k: r[k] for k in ("objp", "objd", "infeasp", "infeasd", "gap", "relgap")})
+                self.info["iter"] = it
+                xs, ys = e.solution()
+                self.itervars["xk"], self.itervars["yk"] = xs[0], ys[0]
+                self.log_iter(False)
+                if cbfun is not None:
+                    cbfun(self.state, self.info)
+                return interrupt_hdl.interrupted
+            eng.solve(max_iters=term_maxiter, block_iters=block, check_every=granularity // block,
+                      tol=term_pd_gap[0], tol_infeas=term_pd_gap[1], use_graph=use_graph, callback=cb)
         torch.cuda.synchronize()
         return self.info
 

@@ -16,23 +16,25 @@ from opymize_amd.tools.synth import rects
 pytestmark = pytest.mark.gpu
 
 
-def _solve_gpu(f, L, lam, precision, iters, gran=None, use_graph=True):
+def _solve_gpu(f, L, lam, precision, iters, gran=None, use_graph=True, fused=False):
     from opymize_amd import lifting
     from opymize_amd.solvers import PDHG
     G, F, A, gam = lifting.sublabel_tv(f, L, lam)
     s = PDHG(G, F, A)
     info = s.solve(steps="precond", precision=precision, term_maxiter=iters, granularity=gran or iters,
-                   term_pd_gap=1e-30, use_graph=use_graph)
+                   term_pd_gap=1e-30, use_graph=use_graph, fused=fused)
     x, y = s.state
     return np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), info, gam
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("L", [2, 16])
-def test_lifting_iterates_match_oracle(gpu, precision, L):
+def test_lifting_iterates_match_oracle(gpu, precision, L, fused):
+    """The generic chain (per-operator kernels) and the fused batched kernels."""
     f = rects(20, 17, 1, seed=11).reshape(20, 17)
     lam, iters = 0.25, 120
-    x, y, info, gam = _solve_gpu(f, L, lam, precision, iters)
+    x, y, info, gam = _solve_gpu(f, L, lam, precision, iters, fused=fused)
     prob, ogam = olf.problem(f, L, lam)
     assert np.allclose(gam, ogam, rtol=0, atol=1e-15)
     xo, yo, io, _ = opd.solve(prob, steps="precond", term_maxiter=iters, granularity=iters, term_pd_gap=1e-30)
@@ -87,3 +89,39 @@ def test_lifting_solution_matches_rof(gpu):
     assert abs(info["relgap"]) < 1e-4
     assert err <= bound
     assert bound < 0.2 * np.linalg.norm(xr - f.ravel())
+
+
+def test_lifting_batch_uneven_sizes_and_labels(gpu):
+    """One batch of instances of uneven image size and label count (2 to 33
+    labels, tiles cut by the image edges) against per-instance oracle runs
+    after a fixed number of iterations, in FP64; every instance's gap as the
+    oracle's."""
+    from opymize_amd.solvers.lift import LiftBatch
+    shapes = [(20, 17), (9, 70), (33, 12), (5, 130), (64, 65)]
+    Ls = [16, 5, 2, 33, 9]
+    lams = [0.25, 0.1, 0.4, 0.2, 0.3]
+    imgs = [rects(H, W, 1, seed=20 + i).reshape(H, W) for i, (H, W) in enumerate(shapes)]
+    bat = LiftBatch(imgs, Ls, lams, precision="double")
+    iters = 60
+    res = bat.solve(max_iters=iters, block_iters=20, check_every=3, tol=1e-30)
+    xs, ys = bat.solution()
+    for b, im in enumerate(imgs):
+        prob, gam = olf.problem(im, Ls[b], lams[b])
+        xo, yo, io, _ = opd.solve(prob, steps="precond", term_maxiter=iters, granularity=iters, term_pd_gap=1e-30)
+        print("instance %d: %s L=%d max |x - xo| %.2e" % (b, shapes[b], Ls[b], np.abs(xs[b] - xo).max()))
+        assert np.abs(xs[b] - xo).max() < 1e-9, b
+        assert np.abs(ys[b] - yo).max() < 1e-9 * max(1.0, np.abs(yo).max()), b
+        assert res[b]["iter"] == iters
+        assert abs(res[b]["gap"] - io["gap"]) < 1e-9 * max(1.0, abs(io["objp"])), (b, res[b]["gap"], io["gap"])
+        assert abs(res[b]["objd"] - io["objd"]) < 1e-9 * max(1.0, abs(io["objp"])), b
+
+
+def test_lifting_batch_stops_certified_instances(gpu):
+    """Instances that certify stop iterating while the others go on; each
+    stops at a check with its gap below the tolerance."""
+    from opymize_amd.solvers.lift import LiftBatch
+    imgs = [rects(12, 10, 1, seed=5).reshape(12, 10), rects(24, 30, 1, seed=6).reshape(24, 30)]
+    bat = LiftBatch(imgs, [2, 8], [0.25, 0.25], precision="double")
+    res = bat.solve(max_iters=20000, block_iters=100, check_every=1, tol=1e-8)
+    assert res[0]["status"] == 1 and abs(res[0]["relgap"]) < 1e-8
+    assert res[0]["iter"] < res[1]["iter"] or res[1]["status"] != 1
