@@ -3,30 +3,39 @@
 Metric: solves/sec to a relative primal-dual gap of 1e-5, certified in FP64,
 for the batched fused PDHG on the learned-warm-start workload: TV-L2
 denoising of 1024x1024 images from the rectangle generator, lambda
-log-uniform in [1, 20], 256 instances per batch.  A step = the trained
-network's prediction of (x, y) for the resident batch, then the batched
-solver from that start until every instance is certified or the iteration
-cap, with a cold restart for any instance whose gap does not improve at the
-first check (--cold: from x = y = 0 instead).  The batched solver runs the
-accelerated steps of Chambolle & Pock (2011, Algorithm 2) on these strongly
-convex instances, restarting an instance's step schedule when its certified
-gap has fallen enough or stopped falling (--no-restart: never; --step-rule
-adaptive: the reference's rule).  After the
-timed steps the other start (cold, or warm with --cold) is solved once more
-for comparison.
+log-uniform in [1, 20], 256 instances per batch.  A step = the batched
+solver from the cold start x = y = 0 until every instance is certified or
+the iteration cap (--warm: from the trained network's prediction of (x, y)
+for the resident batch, with a cold restart for any instance whose gap does
+not improve at the first check).  The cold start is timed by default because
+the warm start saves no iterations at 1e-5 under the restarted schedule
+below and costs its inference; the warm start is solved once more after the
+timed steps and reported beside it (config.other_start).  The batched
+solver runs the accelerated steps of Chambolle & Pock (2011, Algorithm 2) on
+these strongly convex instances, restarting an instance's step schedule when
+its certified gap has fallen enough or stopped falling (--no-restart: never;
+--step-rule adaptive: the reference's rule).
+
+--workload lifting instead solves a batch of 512x512, L = 16 multi-label
+lifting instances (opymize_amd.lifting) to 1e-5 with the fused batched
+kernels (opymize_amd.solvers.lift) and preconditioned steps, against the
+plain single-instance drop-in path (the generic kernel chain).
 
 Printed next to it:
   * the plain single-instance GPU PDHG (the drop-in generic kernel chain of
     PDHG.solve, cold start, the reference's adaptive steps, same cap) on
-    instances spread over the lambda range of the same batch;
+    instances spread over the lambda range of the same batch, and the same
+    instances one at a time through the batched solver's own kernels and
+    step rule (so that ratio isolates the gain of batching);
   * the roofline of the dominant kernel (by default the kernel that runs K
     iterations per launch on LDS-staged tiles; the single-pass iteration
     kernel with --multi 1 or adaptive steps; the dual pass with --passes 2):
     algorithmic bytes of a full-batch launch over the average duration of
     eager full-batch launches, timed with HIP events on the launch stream;
-  * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample.
+  * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample,
+    and a pool of processes running it on as many instances at once.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cold]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--warm] [--workload tv|lifting]
        python bench.py --roofline-only   (only the kernel-timing phase; for counter runs)
 """
 import argparse
@@ -125,10 +134,10 @@ def roofline_phase(bat, reps, rule):
 
 
 def measured_traffic(npx, tile, passes, rule, multi):
-    """HBM bytes per full-batch launch of the dominant kernel from the
-    committed counter runs (FETCH_SIZE and WRITE_SIZE, each scaled by what it
-    reads for a copy of known size), when they were taken on this workload;
-    else None."""
+    """(HBM bytes per full-batch launch of the dominant kernel, where they come
+    from): the committed counter runs (FETCH_SIZE and WRITE_SIZE, each scaled
+    by what it reads for a copy of known size), when they were taken on this
+    workload; else (None, None)."""
     if passes == 2:
         name = "tv_dual_kernel"
     elif multi > 1:
@@ -138,15 +147,17 @@ def measured_traffic(npx, tile, passes, rule, multi):
     name += "_accel" if rule == "accel" else ""
     path = os.path.join(ROOT, "profiles", "%s_tvl2_256x1024_counters.json" % name)
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as fh:
         m = json.load(fh)
     if m["workload"]["pixels"] != npx or list(m["workload"]["tile"]) != list(tile):
-        return None
+        return None, None
     kib = 1024.0
     fetch = m["fetch_kib"] * (m["copy_known_bytes_each_way"] / (m["copy_fetch_kib"] * kib))
     write = m["write_kib"] * (m["copy_known_bytes_each_way"] / (m["copy_write_kib"] * kib))
-    return (fetch + write) * kib
+    return (fetch + write) * kib, ("not measured in this run: FETCH_SIZE + WRITE_SIZE of an earlier rocprofv3 "
+                                   "--pmc run of this kernel and workload, read from %s"
+                                   % os.path.relpath(path, ROOT))
 
 
 def plain_single_instance(bat, specs, n, max_iters):
@@ -179,10 +190,82 @@ def plain_single_instance(bat, specs, n, max_iters):
             "seconds": t_tot, "lams": [round(specs[b]["lam"], 3) for b in pick]}
 
 
+def plain_same_rule(bat, specs, n, max_iters, args):
+    """The batched solver's own step rule and kernels on one instance at a time
+    (TVBatch of a single image, cold start): the same instances as the plain
+    generic leg, so the ratio to it isolates what batching adds."""
+    import torch
+    from opymize_amd.solvers.tv import TVBatch
+    f_all = bat.unpack(bat.f, "x")
+    order = np.argsort([s["lam"] for s in specs])
+    pick = [int(order[int(round(q))]) for q in np.linspace(0, len(order) - 1, n)]
+    solved, iters, t_tot = 0, [], 0.0
+    for b in pick:
+        H, W = bat.shapes[b]
+        one = TVBatch([f_all[b].reshape(H, W)], [specs[b]["lam"]], "l2", precision="float", tile=tuple(args.tile),
+                      passes=args.passes, multi=bat.multi)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = one.solve(steps=args.step_rule, step_bound=0.124, step_factor=args.step_factor, max_iters=max_iters,
+                        block_iters=args.block_iters, tol=1e-5, certify=True, restart=not args.no_restart)
+        torch.cuda.synchronize()
+        t_tot += time.perf_counter() - t
+        solved += res[0]["status"] == 1
+        iters.append(int(res[0]["iter"]))
+        del one
+    return {"value": solved / t_tot, "instances": pick, "certified": int(solved), "iters": iters,
+            "seconds": t_tot}
+
+
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _pool_worker(a):
+    """One process of the CPU pool: the oracle on its own instance for a fixed
+    number of iterations; returns (iterations, seconds)."""
+    f, H, W, lam, rule, n_it = a
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle import pdhg as opd
+    prob = opd.tv_problem(f.reshape(-1, 1), (H, W), lam)
+    t = time.perf_counter()
+    opd.solve(prob, steps=rule, step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
+    return n_it, time.perf_counter() - t
+
+
+def cpu_pool(bat, specs, procs, per_it_1core, budget_s, mean_iters, rule):
+    """The oracle on `procs` processes at once, each on a different instance of
+    the batch, for about budget_s seconds: aggregate iterations per second over
+    the mean iterations per instance of the GPU run."""
+    import multiprocessing as mp
+    f_all = bat.unpack(bat.f, "x")
+    n_it = max(2, int(budget_s / max(per_it_1core, 1e-4)))
+    jobs = [(f_all[b], bat.shapes[b][0], bat.shapes[b][1], specs[b]["lam"], rule, n_it) for b in range(procs)]
+    ctx = mp.get_context("spawn")
+    t = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        out = pool.map(_pool_worker, jobs)
+    wall = time.perf_counter() - t
+    rate = sum(o[0] for o in out) / wall
+    return {"processes": procs, "value": rate / mean_iters, "unit": "solves/s",
+            "sample": "%d processes x %d oracle iterations (one instance each, 1 thread each) in %.1fs wall "
+                      "(process start included), scaled to the mean %d iterations per instance of the GPU run"
+                      % (procs, n_it, wall, mean_iters)}
+
+
 def cpu_baseline(bat, specs, budget_s, gpu_iters, rule):
     """Oracle (float64 numpy/scipy, one thread, same step rule) on instance 0:
-    seconds per iteration over a bounded sample, scaled by the iterations the
-    GPU needed for that instance."""
+    seconds per iteration over a bounded sample, scaled by the mean
+    iterations per instance of the GPU run (gpu_iters)."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     from oracle import pdhg as opd
     f = bat.unpack(bat.f, "x")[0].reshape(-1, 1)
@@ -198,10 +281,191 @@ def cpu_baseline(bat, specs, budget_s, gpu_iters, rule):
         opd.solve(prob, steps=rule, step_bound=0.124, term_maxiter=n_it, granularity=10 ** 9)
         dt = time.perf_counter() - t
     per_it = dt / n_it
+    model, ncpu = _cpu_info()
     return {"value": 1.0 / (per_it * gpu_iters), "unit": "solves/s", "cores": 1, "kind": "port",
+            "cpu_model": model, "cpu_count": ncpu, "seconds_per_iteration": per_it,
             "sample": "oracle (float64 numpy/scipy, %s steps, 1 thread) on instance 0 (%dx%d): %d iterations "
-                      "in %.1fs, scaled to the %d iterations the GPU needed for that instance"
+                      "in %.1fs, scaled to the mean %d iterations per instance of the GPU run"
                       % (rule, H, W, n_it, dt, gpu_iters)}
+
+
+LIFT_SIDE, LIFT_LABELS, LIFT_LAM, LIFT_SEED = 512, 16, 0.25, 4
+LIFT_OMEGA = 6.0  # primal weight of the batched lifting solve (1 = the reference's preconditioner)
+
+
+def lifting_kernels(bat, A, N, k, reps):
+    """Eager full-batch launches of the fused lifting passes and, for scale,
+    of the generic chain's per-pixel product on the matrix cores (r (N, k) ->
+    z (N, 2k), the MatrixMultR of one instance), timed with HIP events on
+    their stream.  Algorithmic bytes of a fused iteration: the primal pass
+    reads U, r, p (2), z (2) and writes U, x_bar's U, r, x_bar's r: 10
+    planes of k values per pixel; the dual pass reads x_bar (2), p (2), z (2)
+    and f and writes p (2), z (2): 10 planes plus f."""
+    import torch
+    from opymize_amd import _hip
+    stream = torch.cuda.current_stream()
+    bat.reset()
+    bat.refresh_grid()
+    Pt, s = _hip.ptr, _hip.stream()
+    ctl = lambda: (Pt(bat.insts), Pt(bat.active_inst), Pt(bat.act_off), Pt(bat.nact), bat.kmax, bat.grid, s)
+    npl = sum(h * w * kk for (h, w), kk in zip(bat.shapes, bat.K))
+    npx = sum(h * w for h, w in bat.shapes)
+    es = 4 if bat.dtype == torch.float32 else 8
+    mm = A.blocks[1][1]
+    r = torch.rand(N * k, dtype=torch.float32, device="cuda")
+    z = torch.zeros(N * 2 * k, dtype=torch.float32, device="cuda")
+    cases = {
+        "lift_primal": (lambda: bat._call("opy_lift_primal" + bat._sfx(), Pt(bat.U), Pt(bat.r), Pt(bat.Ub),
+                                          Pt(bat.rb), Pt(bat.p), Pt(bat.z), *ctl()), 10 * npl * es),
+        "lift_dual": (lambda: bat._call("opy_lift_dual" + bat._sfx(), Pt(bat.Ub), Pt(bat.rb), Pt(bat.p), Pt(bat.z),
+                                        Pt(bat.f), *ctl()), 10 * npl * es + npx * es),
+        "matmul_r_mfma (one instance, generic chain)": (lambda: mm(r, z), N * 3 * k * 4),
+    }
+    out = {}
+    for name, (fn, nbytes) in cases.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": ms, "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    _hip.call("copy", a.dtype, _hip.ptr(a), _hip.ptr(b), a.numel(), _hip.stream())
+    torch.cuda.synchronize()
+    del a, b
+    bat.reset()
+    return out
+
+
+def lifting_main(args):
+    """The multi-label lifting workload: a batch of 512x512, L = 16 instances
+    solved together to relative gap 1e-5 (FP32 iterates, FP64 gap on the GPU)
+    by the fused batched kernels with preconditioned steps and primal weight
+    LIFT_OMEGA, against the plain single-instance drop-in PDHG (the generic
+    kernel chain, the reference's preconditioner)."""
+    import torch
+    from opymize_amd import lifting
+    from opymize_amd.solvers import PDHG
+    from opymize_amd.solvers.lift import LiftBatch
+    from opymize_amd.tools import synth
+    n, L, lam = LIFT_SIDE, LIFT_LABELS, LIFT_LAM
+    B = args.lift_batch
+    imgs = [synth.rects(n, n, 1, seed=LIFT_SEED + b).reshape(n, n) for b in range(B)]
+    N, k = n * n, L - 1
+    bat = LiftBatch(imgs, L, lam, precision="float", omega=LIFT_OMEGA)
+    G, F, A, gam = lifting.sublabel_tv(imgs[0], L, lam)
+    kt = lifting_kernels(bat, A, N, k, args.reps)
+    log("lifting kernels: %s" % ", ".join("%s %.4f ms %.0f GB/s" % (a, v["ms"], v["GBs"]) for a, v in kt.items()))
+    if args.roofline_only:
+        return
+
+    def one_step():
+        bat.reset()
+        return bat.solve(max_iters=args.lifting_cap, block_iters=100, check_every=10, tol=1e-5)
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    certified, iters = 0, []
+    for _ in range(args.steps):
+        res = one_step()
+        certified += sum(r["status"] == 1 for r in res)
+        iters += [r["iter"] for r in res]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = certified / elapsed
+    log("lifting batch: %d/%d certified in %.2fs, iterations %s" % (certified, args.steps * B, elapsed, iters))
+
+    # the plain single-instance drop-in path (generic kernel chain) on instance 0: seconds per iteration
+    # over a bounded sample, times the iterations of its rule on that instance (the fused kernels give the
+    # same iterates up to rounding, so their count is measured there, in one single-instance solve)
+    def generic_rate(weight, nit):
+        s = PDHG(G, F, A)
+        s.solve(steps="precond", precision="float", term_maxiter=200, granularity=200, term_pd_gap=1e-30,
+                fused=False, precond_weight=weight)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s = PDHG(G, F, A)
+        s.solve(steps="precond", precision="float", term_maxiter=nit, granularity=nit, term_pd_gap=1e-30,
+                fused=False, precond_weight=weight)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / nit
+
+    def single_iters(weight, cap):
+        one = LiftBatch(imgs[:1], L, lam, precision="float", omega=weight)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = one.solve(max_iters=cap, block_iters=100, check_every=10, tol=1e-5)[0]
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t
+
+    plain = None
+    if args.plain > 0:
+        r_ref, t_ref = single_iters(1.0, args.lifting_plain_cap)
+        r_w, t_w = single_iters(LIFT_OMEGA, args.lifting_cap)
+        spi = generic_rate(1.0, 3000)
+        plain = {"solves_per_sec": (r_ref["status"] == 1) / (spi * r_ref["iter"]),
+                 "seconds_per_iteration": spi, "iterations": r_ref["iter"], "certified": r_ref["status"] == 1,
+                 "how": "generic chain, reference preconditioner: 3000 timed iterations on instance 0 x the "
+                        "iterations that rule needs there (fused single-instance solve, same iterates up to "
+                        "rounding)",
+                 "single_instance_fused_reference_rule_solves_per_sec": (r_ref["status"] == 1) / t_ref,
+                 "single_instance_fused_same_rule_solves_per_sec": (r_w["status"] == 1) / t_w,
+                 "single_instance_fused_same_rule_iterations": r_w["iter"]}
+        log("lifting plain: %s" % plain)
+    cpu = None
+    if not args.no_cpu:
+        os.environ.setdefault("OMP_NUM_THREADS", "1")
+        from oracle import lifting as olf, pdhg as opd
+        prob, _ = olf.problem(imgs[0], L, lam)
+        n_it = 3
+        t = time.perf_counter()
+        opd.solve(prob, steps="precond", term_maxiter=n_it, granularity=10 ** 9, term_pd_gap=1e-30,
+                  precond_weight=LIFT_OMEGA)
+        dt = time.perf_counter() - t
+        model, ncpu = _cpu_info()
+        it0 = int(round(np.mean(iters)))
+        cpu = {"value": 1.0 / (dt / n_it * it0), "unit": "solves/s", "cores": 1, "kind": "port",
+               "cpu_model": model, "cpu_count": ncpu,
+               "sample": "oracle (float64 numpy/scipy, 1 thread): %d preconditioned iterations of one 512x512, "
+                         "L=16 instance in %.1fs (step set-up included), scaled to the mean %d iterations per "
+                         "instance of the GPU batch" % (n_it, dt, it0)}
+    prim, dual = kt["lift_primal"], kt["lift_dual"]
+    it_ms = prim["ms"] + dual["ms"]
+    it_bytes = prim["bytes"] + dual["bytes"]
+    achieved = it_bytes / (it_ms * 1e-3) / 1e9
+    line = {
+        "metric": "solves/sec to rel PD-gap 1e-5: batched+warm-start vs plain PDHG; %HBM peak",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "multi-label lifting, %d instances of %dx%d (rectangle generator, seeds %d..), "
+                               "L=%d labels, (s - f)^2 data pieces as quadratic-epigraph constraints, TV weight "
+                               "%g, preconditioned steps with primal weight %g, rel gap 1e-5 FP64 on the GPU, cap "
+                               "%d iterations, cold start"
+                               % (B, n, n, LIFT_SEED, L, lam, LIFT_OMEGA, args.lifting_cap),
+                   "path": "fused batched kernels (LiftBatch)", "certified": int(certified),
+                   "instances": int(args.steps * B), "iterations": iters,
+                   "plain_single_instance": plain,
+                   "ratio_vs_plain": (value / plain["solves_per_sec"]) if plain and plain["solves_per_sec"] else None,
+                   "warm_start": "none: a start lifted from the ROF solution (primal, or primal and dual) needed "
+                                 "more iterations than the cold start",
+                   "kernels": kt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "traffic_source": None,
+                     "kernel": "lift_primal_kernel<float,16> + lift_dual_kernel<float,16>: one iteration over the "
+                               "batch, %d B algorithmic (20 planes of k=%d FP32 values per pixel, and f)"
+                               % (it_bytes, k),
+                     "avg_ms": it_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -218,7 +482,18 @@ def main():
     ap.add_argument("--multi", type=int, default=4, choices=[1, 2, 3, 4],
                     help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
-    ap.add_argument("--cold", action="store_true", help="start from x = y = 0 instead of the network's prediction")
+    ap.add_argument("--warm", action="store_true",
+                    help="time the learned warm start (the network's prediction of x, y) instead of the cold start")
+    ap.add_argument("--cold", action="store_true", help="start from x = y = 0 (the default)")
+    ap.add_argument("--workload", default="tv", choices=["tv", "lifting"],
+                    help="tv: the batched TV-L2 solve; lifting: the 512x512, L=16 multi-label lifting model")
+    ap.add_argument("--lifting-cap", type=int, default=200000, help="iteration cap of the lifting solve")
+    ap.add_argument("--lift-batch", type=int, default=8, help="instances of the lifting batch")
+    ap.add_argument("--lifting-plain-cap", type=int, default=400000,
+                    help="iteration cap of the reference-rule single-instance lifting solve")
+    ap.add_argument("--plain-same-rule", type=int, default=6,
+                    help="instances for the single-instance fused baseline with the batched step rule")
+    ap.add_argument("--pool", type=int, default=16, help="processes of the CPU pool baseline (0: skip)")
     ap.add_argument("--step-factor", type=float, default=4.0,
                     help="initial tau/sigma split: tau = sqrt(bound * factor), sigma = sqrt(bound / factor)")
     ap.add_argument("--no-compare", action="store_true", help="skip the solve from the other start")
@@ -241,6 +516,8 @@ def main():
     from opymize_amd import _hip
     _hip.require_device()
     _hip.load()
+    if args.workload == "lifting":
+        return lifting_main(args)
     bat, specs = build_batch(args)
     npx = sum(h * w for h, w in bat.shapes)
     lams = [s["lam"] for s in specs]
@@ -262,7 +539,7 @@ def main():
                          max_iters=args.max_iters, block_iters=args.block_iters, tol=1e-5, certify=True,
                          fallback=warm, restart=not args.no_restart)
 
-    warm = not args.cold
+    warm = bool(args.warm) and not args.cold
 
     for _ in range(args.warmup):
         one_step(warm)
@@ -294,7 +571,10 @@ def main():
         dt = time.perf_counter() - t1
         c = int(sum(r["status"] == 1 for r in res))
         other = {"start": "cold" if warm else "warm", "certified": c, "seconds": dt, "solves_per_sec": c / dt,
-                 "iterations_median": float(np.median([r["iter"] for r in res]))}
+                 "iterations_median": float(np.median([r["iter"] for r in res])),
+                 "iterations_max": int(max(r["iter"] for r in res)),
+                 "fallbacks": len(bat.fallbacks) if not warm else None}
+        other["ratio_to_timed"] = other["solves_per_sec"] / value if value > 0 else None
         log("batched (%s start, for comparison): %d/%d certified in %.2fs" % (other["start"], c, args.batch, dt))
 
     kt = roofline_phase(bat, args.reps, args.step_rule)
@@ -319,13 +599,23 @@ def main():
     if plain:
         log("plain single-instance: %d/%d certified, %.3f solves/s, iterations %s, lambdas %s"
             % (plain["certified"], len(plain["instances"]), plain["value"], plain["iters"], plain["lams"]))
+    same = plain_same_rule(bat, specs, args.plain_same_rule, args.max_iters, args) \
+        if args.plain_same_rule > 0 else None
+    if same:
+        log("single-instance fused, same rule: %d/%d certified, %.2f solves/s, iterations %s"
+            % (same["certified"], len(same["instances"]), same["value"], same["iters"]))
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bat, specs, args.cpu_budget, int(iters[0]), args.step_rule)
+        cpu = cpu_baseline(bat, specs, args.cpu_budget, int(round(iters.mean())), args.step_rule)
         log("cpu oracle: %s" % cpu["sample"])
+        if args.pool > 0:
+            cpu["pool"] = cpu_pool(bat, specs, min(args.pool, args.batch), cpu["seconds_per_iteration"],
+                                   args.cpu_budget, int(round(iters.mean())), args.step_rule)
+            log("cpu pool: %s" % cpu["pool"]["sample"])
 
     from opymize_amd.solvers import tv as tvmod
+    traffic = measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch)
     line = {
         "metric": "solves/sec to rel PD-gap 1e-5: batched+warm-start vs plain PDHG; %HBM peak",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -349,10 +639,15 @@ def main():
                    "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max()),
                    "plain_single_instance_solves_per_sec": plain["value"] if plain else None,
                    "plain_instances": len(plain["instances"]) if plain else 0,
-                   "ratio_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None},
+                   "plain_certified": plain["certified"] if plain else 0,
+                   "ratio_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None,
+                   "single_instance_same_rule_solves_per_sec": same["value"] if same else None,
+                   "single_instance_same_rule_certified": same["certified"] if same else 0,
+                   "ratio_vs_single_instance_same_rule":
+                       (value / same["value"]) if same and same["value"] > 0 else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch),
+                     "traffic": traffic[0], "traffic_source": traffic[1],
                      "kernel": "%s: %d B/pixel algorithmic x %d pixels per launch" % (kdesc, kbytes, npx),
                      "avg_ms": kms, "ms_per_iteration": kms / k_launch,
                      "launch_ms": {k: v for k, v in kt.items() if k != "grid"},

@@ -449,6 +449,7 @@ typedef struct {
   double lam;     /* TV weight */
   double g0, h;   /* first label, label spacing */
   double tau_r;   /* primal step of the piece weights: min over l of 1/(|g_l| + h l + 1) */
+  double omega;   /* primal weight: primal steps times omega, dual steps over omega (1: the reference's) */
 } opy_lift_inst_t;
 /* primal pass, in place: U <- clip(U - tau K^T y), r <- simplex(r - tau K^T y); Ub, rb <- 2 x+ - x.
  * kmax = the largest K of the batch; `grid` blocks walk the running tiles. */

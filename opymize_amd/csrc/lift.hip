@@ -141,8 +141,8 @@ __global__ void __launch_bounds__(256) lift_primal_kernel(T* __restrict__ U, T* 
     const T h = (T)I.h;
     // column sums of |K|: the gradient's entries of U_i at this pixel, and h in a_i
     const int cnt = (q.row < I.H - 1) + (q.row > 0) + (q.col < I.W - 1) + (q.col > 0);
-    const T tau_u = T(1) / ((T)cnt + h);
-    const T tau_r = (T)I.tau_r;
+    const T tau_u = (T)I.omega / ((T)cnt + h);
+    const T tau_r = (T)(I.omega * I.tau_r);
     T rv[KMAX], r0[KMAX];
     T P = T(0);  // sum of nu_l for l < i
 #pragma unroll
@@ -195,6 +195,8 @@ __global__ void __launch_bounds__(256) lift_dual_kernel(const T* __restrict__ Ub
     const int64_t pix = (int64_t)q.row * I.W + q.col;
     const int K = I.K;
     const T h = (T)I.h, rad = (T)(I.lam * I.h);
+    const T iw = (T)(1.0 / I.omega);
+    const T sp = T(0.5) * iw;  // gradient rows: two entries of 1 in absolute value
     const bool dn = q.row < I.H - 1, rt = q.col < I.W - 1;
     const T fv = f[I.off + pix];
     const T s1 = T(-2) * fv, s2 = -fv * fv;  // epi{nu^2/4 + f nu} = epi{0.25 |nu - s1|^2 + s2}
@@ -208,10 +210,10 @@ __global__ void __launch_bounds__(256) lift_dual_kernel(const T* __restrict__ Ub
         const int64_t o = I.offk + i * N + pix;
         const int64_t o2 = 2 * I.offk + 2 * i * N + pix;
         const T ub = Ub[o];
-        // gradient rows: two entries of 1 in absolute value inside, none on the last row / column
+        // no gradient entries on the last row / column: p stays there
         T q0 = p[o2], q1 = p[o2 + N];
-        if (dn) q0 += T(0.5) * (Ub[o + I.W] - ub);
-        if (rt) q1 += T(0.5) * (Ub[o + 1] - ub);
+        if (dn) q0 += sp * (Ub[o + I.W] - ub);
+        if (rt) q1 += sp * (Ub[o + 1] - ub);
         const T n2 = q0 * q0 + q1 * q1;
         if (n2 > rad * rad) {
           const T sc = rad / sqrt(n2);
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(256) lift_dual_kernel(const T* __restrict__ Ub
         const T a = h * ub + gi * rbv[i] - h * S;
         S += rbv[i];
         const T rowsum = h + fabs(gi) + h * (T)(K - 1 - i);
-        const T sz = rowsum > T(1) ? T(1) / rowsum : T(1);
+        const T sz = (rowsum > T(1) ? T(1) / rowsum : T(1)) * iw;
         T v1 = z[o2] + sz * a - s1;
         T v2 = z[o2 + N] - sz * rbv[i] - s2;
         quadepi_project1<T>(v1, v2, T(0.5));

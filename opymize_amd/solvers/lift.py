@@ -27,7 +27,7 @@ class LiftInst(ctypes.Structure):
     _fields_ = [("off", ctypes.c_int64), ("offk", ctypes.c_int64), ("H", ctypes.c_int32), ("W", ctypes.c_int32),
                 ("K", ctypes.c_int32), ("tile0", ctypes.c_int32), ("ntiles", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("lam", ctypes.c_double), ("g0", ctypes.c_double),
-                ("h", ctypes.c_double), ("tau_r", ctypes.c_double)]
+                ("h", ctypes.c_double), ("tau_r", ctypes.c_double), ("omega", ctypes.c_double)]
 
 
 P, I, L, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -65,9 +65,12 @@ def exported_names():
 class LiftBatch(object):
     """images: list of 2-D arrays (H_b, W_b); L: labels per instance (scalar
     or list, 2..33); lam: TV weight per instance; lo, hi: label range per
-    instance (default each image's min and max)."""
+    instance (default each image's min and max); omega: primal weight per
+    instance, the diagonal steps become omega*T and Sigma/omega (the
+    convergence condition |Sigma^1/2 K T^1/2| <= 1 does not change; 1 gives
+    the reference's preconditioner)."""
 
-    def __init__(self, images, L, lam, lo=None, hi=None, precision="float"):
+    def __init__(self, images, L, lam, lo=None, hi=None, precision="float", omega=1.0):
         _hip.require_device()
         _bind()
         imgs = [np.asarray(im, dtype=np.float64) for im in images]
@@ -80,6 +83,10 @@ class LiftBatch(object):
         lam = np.broadcast_to(np.asarray(lam, dtype=np.float64), (self.B,)).copy()
         if np.any(lam <= 0):
             raise ValueError("lam must be positive")
+        omega = np.broadcast_to(np.asarray(omega, dtype=np.float64), (self.B,)).copy()
+        if np.any(~(omega > 0)):
+            raise ValueError("omega must be positive")
+        self.omega = omega
         los = [im.min() if lo is None else np.broadcast_to(lo, (self.B,))[b] for b, im in enumerate(imgs)]
         his = [im.max() if hi is None else np.broadcast_to(hi, (self.B,))[b] for b, im in enumerate(imgs)]
         self.gam = [labels(Ls[b], los[b], his[b]) for b in range(self.B)]
@@ -102,6 +109,7 @@ class LiftBatch(object):
             rec.lam, rec.g0, rec.h = float(lam[b]), float(g[0]), h
             # column sums of |K| for r_l: |g_l| + h*l (a_i, i < l) + 1 (the pair's -r_l); the smallest step
             rec.tau_r = float(np.min(1.0 / (np.abs(g[:k]) + h * np.arange(k) + 1.0)))
+            rec.omega = float(omega[b])
             self.offs.append(off)
             self.offks.append(offk)
             self.ntiles.append(nt)
@@ -185,6 +193,31 @@ class LiftBatch(object):
         """x = y = 0 for every instance, all running."""
         for t in (self.U, self.r, self.Ub, self.rb, self.p, self.z):
             t.zero_()
+        self.checks.zero_()
+        self.iter_counter.zero_()
+        self._compact(0)
+
+    def set_start(self, xs=None, ys=None):
+        """Start values per instance in the layout of solution() (x = (U, r),
+        y = (p (N*k, 3), z (N*k, 2)); None keeps zeros); all instances running."""
+        U = np.zeros(self.n_planes)
+        r = np.zeros(self.n_planes)
+        p = np.zeros(2 * self.n_planes)
+        z = np.zeros(2 * self.n_planes)
+        for b, (H, W) in enumerate(self.shapes):
+            N, k, ok = H * W, self.K[b], self.offks[b]
+            if xs is not None and xs[b] is not None:
+                x = np.asarray(xs[b], dtype=np.float64).ravel()
+                U[ok:ok + N * k] = x[:N * k].reshape(N, k).T.ravel()
+                r[ok:ok + N * k] = x[N * k:2 * N * k].reshape(N, k).T.ravel()
+            if ys is not None and ys[b] is not None:
+                y = np.asarray(ys[b], dtype=np.float64).ravel()
+                p3 = y[:3 * N * k].reshape(N, k, 3)
+                zz = y[3 * N * k:5 * N * k].reshape(N, k, 2)
+                p[2 * ok:2 * ok + 2 * N * k] = p3[:, :, :2].transpose(1, 2, 0).ravel()
+                z[2 * ok:2 * ok + 2 * N * k] = zz.transpose(1, 2, 0).ravel()
+        for t, v in ((self.U, U), (self.r, r), (self.p, p), (self.z, z)):
+            t.copy_(torch.as_tensor(v, dtype=self.dtype))
         self.checks.zero_()
         self.iter_counter.zero_()
         self._compact(0)

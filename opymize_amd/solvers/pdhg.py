@@ -142,7 +142,7 @@ class PDHG(object):
         return self._gap(d["xk"], d["ygradk"], d["xgradk"], d["yk"])
 
     # -- steps -------------------------------------------------------------
-    def prepare_stepsizes(self, step_bound, step_factor, steps):
+    def prepare_stepsizes(self, step_bound, step_factor, steps, precond_weight=1.0):
         i, c, d = self.itervars, self.constvars, self._dev
         dtype = d["xk"].dtype
         if steps == "precond":
@@ -162,6 +162,15 @@ class PDHG(object):
             for vec, fct in ((ysigma, self.f.conj), (xtau, self.g)):
                 for off, nb, w in fct.precond_blocks():
                     _hip.call("block_min", dtype, _hip.addr(vec, off), nb, w, _hip.stream())
+            if precond_weight != 1.0:
+                # primal weight w (no reference counterpart): steps w*T and Sigma/w leave
+                # |Sigma^1/2 K T^1/2| unchanged, so convergence holds for every w > 0
+                if not precond_weight > 0:
+                    raise ValueError("precond_weight must be positive")
+                _hip.call("scale", dtype, _hip.ptr(xtau), _hip.ptr(xtau), xtau.numel(), None, float(precond_weight),
+                          0, _hip.stream())
+                _hip.call("scale", dtype, _hip.ptr(ysigma), _hip.ptr(ysigma), ysigma.numel(), None,
+                          1.0 / float(precond_weight), 0, _hip.stream())
             d["xtau"], d["ysigma"] = xtau, ysigma
             self.gprox = self.g.prox(xtau)
             self.fconjprox = self.f.conj.prox(ysigma)
@@ -296,14 +305,15 @@ class PDHG(object):
     # -- solve -------------------------------------------------------------
     def solve(self, continue_at=None, precision="double", steps="const", step_bound=None, step_factor=1.0,
               term_pd_gap=1e-5, term_pd_res=None, term_maxiter=int(5e4), granularity=5000, use_gpu=True,
-              cbfun=None, use_graph=True, fused="auto"):
+              cbfun=None, use_graph=True, fused="auto", precond_weight=1.0):
         if not use_gpu:
             raise ValueError("opymize_amd implements the GPU path of PDHG (use_gpu=True)")
         _hip.require_device()
         lift = getattr(self.linop, "lifting_model", None)
         if fused and lift is not None and steps == "precond" and term_pd_res is None and term_maxiter > 0 \
                 and continue_at is None:
-            return self._solve_fused_lift(lift, precision, term_pd_gap, term_maxiter, granularity, cbfun, use_graph)
+            return self._solve_fused_lift(lift, precision, term_pd_gap, term_maxiter, granularity, cbfun, use_graph,
+                                          precond_weight)
         if fused and term_pd_res is None and term_maxiter > 0:
             model = fused_tv_model(self.g, self.f, self.linop)
             if model is not None:
@@ -354,7 +364,7 @@ class PDHG(object):
                 self.info["epspd"] = (term_pd_gap[0],) * 4
 
         c["theta"] = 1.0
-        self.prepare_stepsizes(step_bound, step_factor, steps)
+        self.prepare_stepsizes(step_bound, step_factor, steps, precond_weight)
         if "adaptive" in c:
             p = _hip.AdaptParams()
             p.eta, p.Delta = c["eta"], c["Delta"]
@@ -473,14 +483,15 @@ class PDHG(object):
         torch.cuda.synchronize()
         return self.info
 
-    def _solve_fused_lift(self, model, precision, term_pd_gap, term_maxiter, granularity, cbfun, use_graph):
+    def _solve_fused_lift(self, model, precision, term_pd_gap, term_maxiter, granularity, cbfun, use_graph,
+                          precond_weight=1.0):
         """The multi-label lifting model of opymize_amd.lifting through the fused
         kernels (same iterates as the generic chain with steps="precond")."""
         from opymize_amd.solvers.lift import LiftBatch
         if not isinstance(term_pd_gap, tuple):
             term_pd_gap = (term_pd_gap, term_pd_gap)
         eng = LiftBatch([model["f"]], model["L"], model["lam"], lo=model["lo"], hi=model["hi"],
-                        precision=precision)
+                        precision=precision, omega=precond_weight)
         block = 100 if granularity % 100 == 0 else granularity
         self.constvars = {"theta": 1.0, "precond": True, "fused": True}
         logging.info("Fused lifting kernels (%d labels, preconditioned steps)" % model["L"])

@@ -113,8 +113,10 @@ def pd_gap(prob, x, Ax, ATy, y):
 
 
 def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-5, term_maxiter=1000,
-          granularity=5000, x0=None, y0=None, term_pd_res=None, history=False, restart=None):
+          granularity=5000, x0=None, y0=None, term_pd_res=None, history=False, restart=None, precond_weight=1.0):
     """Iterate exactly as the reference's CPU path; returns (x, y, info, hist).
+    precond_weight (preconditioned steps only, no reference counterpart)
+    scales the primal steps by w and the dual steps by 1/w.
     hist (if requested) holds per-iteration (tau, sigma, alpha, resp, resd).
 
     restart = (beta, min_iters), accelerated steps on a strongly convex G
@@ -147,7 +149,8 @@ def solve(prob, steps="const", step_bound=None, step_factor=1.0, term_pd_gap=1e-
         xtau[xtau > eps] = 1.0 / xtau[xtau > eps]
         ysig = block_min(ysig, prob.yblocks)
         xtau = block_min(xtau, prob.xblocks)
-        tau, sigma = xtau, ysig
+        # primal weight w: steps w*T and Sigma/w (|Sigma^1/2 K T^1/2| is unchanged; w = 1 is the reference's)
+        tau, sigma = xtau * precond_weight, ysig / precond_weight
     else:
         bnd = step_bound
         if bnd is None:

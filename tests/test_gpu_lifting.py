@@ -125,3 +125,23 @@ def test_lifting_batch_stops_certified_instances(gpu):
     res = bat.solve(max_iters=20000, block_iters=100, check_every=1, tol=1e-8)
     assert res[0]["status"] == 1 and abs(res[0]["relgap"]) < 1e-8
     assert res[0]["iter"] < res[1]["iter"] or res[1]["status"] != 1
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_lifting_primal_weight(gpu, fused):
+    """precond_weight w scales the primal steps by w and the dual ones by 1/w,
+    in the generic chain and the fused kernels alike (FP64 against the oracle)."""
+    from opymize_amd import lifting
+    from opymize_amd.solvers import PDHG
+    f = rects(18, 21, 1, seed=13).reshape(18, 21)
+    G, F, A, gam = lifting.sublabel_tv(f, 12, 0.25)
+    s = PDHG(G, F, A)
+    s.solve(steps="precond", precision="double", term_maxiter=80, granularity=80, term_pd_gap=1e-30, fused=fused,
+            precond_weight=6.0)
+    x, y = s.state
+    prob, _ = olf.problem(f, 12, 0.25)
+    xo, yo, io, _ = opd.solve(prob, steps="precond", term_maxiter=80, granularity=80, term_pd_gap=1e-30,
+                              precond_weight=6.0)
+    xo1, _, _, _ = opd.solve(prob, steps="precond", term_maxiter=80, granularity=80, term_pd_gap=1e-30)
+    assert np.abs(x - xo).max() < 1e-9 and np.abs(y - yo).max() < 1e-9 * max(1.0, np.abs(yo).max())
+    assert np.abs(xo - xo1).max() > 1e-3  # the weight changes the iterates
