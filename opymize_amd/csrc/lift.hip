@@ -119,6 +119,11 @@ __device__ inline T kt_u(const T* __restrict__ p, int64_t o2, int64_t N, int W, 
   return v;
 }
 
+// Both passes read a pixel's planes in chunks of kChunk: every load of a chunk
+// is issued before its arithmetic (planes past K re-read plane K-1, whose
+// results are not stored), so that a wavefront keeps many loads in flight.
+constexpr int kChunk = 8;
+
 // ---- primal pass --------------------------------------------------------
 template <typename T, int KMAX>
 __global__ void __launch_bounds__(256) lift_primal_kernel(T* __restrict__ U, T* __restrict__ r, T* __restrict__ Ub,
@@ -140,28 +145,46 @@ __global__ void __launch_bounds__(256) lift_primal_kernel(T* __restrict__ U, T* 
     const int K = I.K;
     const T h = (T)I.h;
     // column sums of |K|: the gradient's entries of U_i at this pixel, and h in a_i
-    const int cnt = (q.row < I.H - 1) + (q.row > 0) + (q.col < I.W - 1) + (q.col > 0);
+    const bool dn = q.row < I.H - 1, up = q.row > 0, rt = q.col < I.W - 1, lf = q.col > 0;
+    const int cnt = (int)dn + (int)up + (int)rt + (int)lf;
     const T tau_u = (T)I.omega / ((T)cnt + h);
     const T tau_r = (T)(I.omega * I.tau_r);
+    // divergence weights: p at this pixel enters with -1 unless on the last row / column,
+    // p at the previous row / column with +1 if there is one (else a harmless read of this pixel)
+    const T wd = dn ? T(1) : T(0), wr = rt ? T(1) : T(0), wu = up ? T(1) : T(0), wl = lf ? T(1) : T(0);
+    const int64_t du = up ? I.W : 0, dl = lf ? 1 : 0;
     T rv[KMAX], r0[KMAX];
     T P = T(0);  // sum of nu_l for l < i
 #pragma unroll
-    for (int i = 0; i < KMAX; ++i) {
-      rv[i] = T(0);
-      r0[i] = T(0);
-      if (i < K) {
+    for (int c0 = 0; c0 < KMAX; c0 += kChunk) {
+      T nu[kChunk], sv[kChunk], uv[kChunk], p0[kChunk], p1[kChunk], pu[kChunk], pl[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int i = min(c0 + j, K - 1);
         const int64_t o = I.offk + i * N + pix;
         const int64_t o2 = 2 * I.offk + 2 * i * N + pix;
-        const T nu = z[o2], s = z[o2 + N];
-        const T u = U[o];
-        const T un = clamp01(u - tau_u * kt_u(p, o2, N, I.W, q.row, q.col, I.H, h, nu));
-        U[o] = un;
-        Ub[o] = T(2) * un - u;
+        nu[j] = z[o2];
+        sv[j] = z[o2 + N];
+        uv[j] = U[o];
+        r0[c0 + j] = r[o];
+        p0[j] = p[o2];
+        p1[j] = p[o2 + N];
+        pu[j] = p[o2 - du];
+        pl[j] = p[o2 + N - dl];
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int i = c0 + j;
+        const T ktu = h * nu[j] - wd * p0[j] + wu * pu[j] - wr * p1[j] + wl * pl[j];
+        const T un = clamp01(uv[j] - tau_u * ktu);
         const T gi = (T)(I.g0 + i * I.h);
-        const T ri = r[o];
-        r0[i] = ri;
-        rv[i] = ri - tau_r * (gi * nu - s - h * P);
-        P += nu;
+        rv[i] = r0[i] - tau_r * (gi * nu[j] - sv[j] - h * P);
+        if (i < K) {
+          const int64_t o = I.offk + i * N + pix;
+          U[o] = un;
+          Ub[o] = T(2) * un - uv[j];
+          P += nu[j];
+        }
       }
     }
     simplex_regs<T, KMAX>(rv, K);
@@ -196,40 +219,55 @@ __global__ void __launch_bounds__(256) lift_dual_kernel(const T* __restrict__ Ub
     const int K = I.K;
     const T h = (T)I.h, rad = (T)(I.lam * I.h);
     const T iw = (T)(1.0 / I.omega);
-    const T sp = T(0.5) * iw;  // gradient rows: two entries of 1 in absolute value
     const bool dn = q.row < I.H - 1, rt = q.col < I.W - 1;
+    // gradient rows: two entries of 1 in absolute value, none on the last row / column (p stays)
+    const T sd = dn ? T(0.5) * iw : T(0), sr = rt ? T(0.5) * iw : T(0);
+    const int64_t dd = dn ? I.W : 0, dr = rt ? 1 : 0;
     const T fv = f[I.off + pix];
     const T s1 = T(-2) * fv, s2 = -fv * fv;  // epi{nu^2/4 + f nu} = epi{0.25 |nu - s1|^2 + s2}
     T rbv[KMAX];
 #pragma unroll
-    for (int i = 0; i < KMAX; ++i) rbv[i] = i < K ? rb[I.offk + i * N + pix] : T(0);
+    for (int i = 0; i < KMAX; ++i) rbv[i] = rb[I.offk + min(i, K - 1) * N + pix];
     T S = T(0);  // sum of rb_l for l > i
 #pragma unroll
-    for (int i = KMAX - 1; i >= 0; --i) {
-      if (i < K) {
+    for (int c1 = KMAX; c1 > 0; c1 -= kChunk) {
+      T ub[kChunk], ud[kChunk], ur[kChunk], q0[kChunk], q1[kChunk], nu[kChunk], sv[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int i = min(c1 - 1 - j, K - 1);
         const int64_t o = I.offk + i * N + pix;
         const int64_t o2 = 2 * I.offk + 2 * i * N + pix;
-        const T ub = Ub[o];
-        // no gradient entries on the last row / column: p stays there
-        T q0 = p[o2], q1 = p[o2 + N];
-        if (dn) q0 += sp * (Ub[o + I.W] - ub);
-        if (rt) q1 += sp * (Ub[o + 1] - ub);
-        const T n2 = q0 * q0 + q1 * q1;
+        ub[j] = Ub[o];
+        ud[j] = Ub[o + dd];
+        ur[j] = Ub[o + dr];
+        q0[j] = p[o2];
+        q1[j] = p[o2 + N];
+        nu[j] = z[o2];
+        sv[j] = z[o2 + N];
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int i = c1 - 1 - j;
+        if (i >= K) continue;
+        const int64_t o2 = 2 * I.offk + 2 * i * N + pix;
+        T a0 = q0[j] + sd * (ud[j] - ub[j]);
+        T a1 = q1[j] + sr * (ur[j] - ub[j]);
+        const T n2 = a0 * a0 + a1 * a1;
         if (n2 > rad * rad) {
           const T sc = rad / sqrt(n2);
-          q0 *= sc;
-          q1 *= sc;
+          a0 *= sc;
+          a1 *= sc;
         }
-        p[o2] = q0;
-        p[o2 + N] = q1;
+        p[o2] = a0;
+        p[o2 + N] = a1;
         // pair rows: |h| + |g_i| + h (k-1-i) for nu, 1 for s; the pair's step is the smaller
         const T gi = (T)(I.g0 + i * I.h);
-        const T a = h * ub + gi * rbv[i] - h * S;
+        const T a = h * ub[j] + gi * rbv[i] - h * S;
         S += rbv[i];
         const T rowsum = h + fabs(gi) + h * (T)(K - 1 - i);
         const T sz = (rowsum > T(1) ? T(1) / rowsum : T(1)) * iw;
-        T v1 = z[o2] + sz * a - s1;
-        T v2 = z[o2 + N] - sz * rbv[i] - s2;
+        T v1 = nu[j] + sz * a - s1;
+        T v2 = sv[j] - sz * rbv[i] - s2;
         quadepi_project1<T>(v1, v2, T(0.5));
         z[o2] = v1 + s1;
         z[o2 + N] = v2 + s2;

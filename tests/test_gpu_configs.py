@@ -117,6 +117,50 @@ def test_batched_uneven_512(gpu):
           % (len(imgs), npx / 1e6, k, worst))
 
 
+def test_batched_uneven_512_certifies_every_instance(gpu):
+    """The uneven TV-L1 + TV-L2 batch (512 instances, sides 64..1024, seed 3) on
+    the path the design recommends: accelerated steps with restarts for the
+    TV-L2 instances, the TV-L1 instances on constant steps from their own
+    split (step_factor_l1 = 1/256), tiles and iterations per launch from the
+    autotuner, FP32, checks every 48 iterations.  Every instance is certified
+    at relative gap 1e-5 within 20000 iterations (the TV-L1 instances with
+    lambda close to 1 need 10000-15000), and the six smallest instances agree
+    with the oracle run for the same number of iterations with the same rule."""
+    from opymize_amd.solvers.tv import TVBatch
+    from opymize_amd.solvers import autotune
+    from opymize_amd.tools import synth
+    specs = synth.batch_uneven(512, seed=3)
+    imgs = synth.images(specs)
+    lams = [s["lam"] for s in specs]
+    data = [s["data"] for s in specs]
+    bat = TVBatch(imgs, lams, data, precision="float", tile=(64, 6), multi=4)
+    best = autotune.tune(bat, steps="accel")
+    print("uneven batch: tuned %s" % (best,), flush=True)
+    res = bat.solve(steps="accel", step_bound=0.124, step_factor=4.0, step_factor_l1=1.0 / 256, max_iters=20000,
+                    block_iters=48, tol=1e-5, certify=True)
+    assert all(np.isfinite(r["gap"]) for r in res)
+    nl1 = sum(d == "l1" for d in data)
+    cert = sum(r["status"] == 1 for r in res)
+    print("uneven batch: %d / 512 certified (TV-L1 %d, TV-L2 %d), iterations max %d"
+          % (cert, sum(r["status"] == 1 for r, d in zip(res, data) if d == "l1"),
+             sum(r["status"] == 1 for r, d in zip(res, data) if d == "l2"), max(r["iter"] for r in res)))
+    assert cert == 512 and nl1 > 200
+    assert all(abs(r["relgap_cert"]) < 1e-5 for r in res)
+    xs, _ = bat.solution()
+    for b in np.argsort([im.size for im in imgs])[:6]:
+        H, W = imgs[b].shape
+        prob = opd.tv_problem(imgs[b].reshape(-1, 1), (H, W), lams[b], data[b])
+        it = res[b]["iter"]
+        xo, _, io, _ = opd.solve(prob, steps="accel", step_bound=0.124,
+                                 step_factor=(1.0 / 256) if data[b] == "l1" else 4.0, term_maxiter=it,
+                                 granularity=48, term_pd_gap=1e-30, restart=(0.5, 96))
+        e = np.abs(xs[b] - xo).max()
+        print("instance %d (%dx%d, %s, lambda %.2f): %d iterations, max |x - x_oracle| = %.2e"
+              % (b, H, W, data[b], lams[b], it, e))
+        assert io["iter"] == it
+        assert e < 2e-3, (b, e)
+
+
 def test_warm_start_batch_256x1024(gpu):
     from opymize_amd.solvers.tv import TVBatch
     from opymize_amd.tools import synth
