@@ -302,7 +302,7 @@ class WarmStart(object):
                 if fused and stride is not None and batch.dtype == torch.float32:
                     self.net.forward_packed(_hip.ptr(f_all[offs[0]:]), stride, loglam, n, H, W,
                                             _hip.ptr(batch.x[0][offs[0]:]), _hip.ptr(batch.y[0][2 * offs[0]:]))
-                    self._project(batch, offs, stride, HW)
+                    self._project(batch, sel, offs, stride, HW)
                     continue
                 inp = self._inp.get((n, H, W))
                 if inp is None:
@@ -324,15 +324,17 @@ class WarmStart(object):
                     for j, off in enumerate(offs):
                         batch.x[0][off:off + HW].copy_(xv[j])
                         batch.y[0][2 * off:2 * off + 2 * HW].copy_(yv[j])
-                self._project(batch, offs, stride, HW)
+                self._project(batch, sel, offs, stride, HW)
 
-    def _project(self, batch, offs, stride, HW):
-        """Feasible dual start: every pixel's pair of y onto |y| <= tvw."""
+    def _project(self, batch, sel, offs, stride, HW):
+        """Feasible dual start: every pixel's pair of y onto |y| <= tvw of its
+        own instance (batch.tvw); the network predicts for self.tvw."""
         n = len(offs)
-        if stride == HW:
+        radii = [float(batch.tvw[b]) for b in sel]
+        if stride == HW and len(set(radii)) == 1:
             _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * offs[0]:2 * offs[0] + 2 * n * HW]),
-                      n * HW, 1, 2, float(self.tvw), 0, _hip.stream())
+                      n * HW, 1, 2, radii[0], 0, _hip.stream())
         else:
-            for off in offs:
+            for off, rad in zip(offs, radii):
                 _hip.call("l1norms_proj", batch.dtype, _hip.ptr(batch.y[0][2 * off:2 * off + 2 * HW]), HW,
-                          1, 2, float(self.tvw), 0, _hip.stream())
+                          1, 2, rad, 0, _hip.stream())

@@ -210,3 +210,27 @@ def test_fused_warm_start_io_equals_encoded_path(gpu, sizes):
     np.testing.assert_array_equal(out[True][0], out[False][0])
     np.testing.assert_array_equal(out[True][1], out[False][1])
     assert np.abs(out[True][0]).max() > 0.1
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_warm_start_dual_lies_in_each_instance_ball(gpu, fused):
+    """The predicted dual start is projected onto |y| <= tvw of each instance
+    (the batch's own TV weights, here unlike the network's 1), so every
+    instance starts dual-feasible."""
+    import os
+    from opymize_amd.solvers.tv import TVBatch
+    from opymize_amd.tools import synth
+    from opymize_amd.warmstart.cnn import ConvNet, WarmStart
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "opymize_amd", "warmstart",
+                        "weights_tv_l2.npz")
+    imgs = [synth.rects(40, 56, seed=70 + i) for i in range(4)]
+    lams = [1.5, 4.0, 9.0, 16.0]
+    tvw = [0.05, 0.05, 2.0, 0.3]
+    bat = TVBatch(imgs, lams, "l2", tvw=tvw, precision="float", tile=(64, 6))
+    WarmStart(ConvNet.load(path), chunk=4).predict_into(bat, lams, fused=fused)
+    _, ys = bat.solution()
+    for b in range(4):
+        n = np.linalg.norm(ys[b].reshape(-1, 2), axis=1)
+        assert n.max() <= tvw[b] * (1 + 1e-6), (b, n.max(), tvw[b])
+    # the small radius is active: the network's output exceeds it somewhere
+    assert np.linalg.norm(ys[0].reshape(-1, 2), axis=1).max() > 0.9 * tvw[0]
