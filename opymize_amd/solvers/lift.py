@@ -71,11 +71,11 @@ class LiftBatch(object):
     the reference's preconditioner)."""
 
     def __init__(self, images, L, lam, lo=None, hi=None, precision="float", omega=1.0):
-        _hip.require_device()
-        _bind()
         imgs = [np.asarray(im, dtype=np.float64) for im in images]
         if not imgs or any(im.ndim != 2 for im in imgs):
             raise ValueError("LiftBatch needs a non-empty list of 2-D images")
+        if not all(np.isfinite(im).all() for im in imgs) or any(im.size == 0 for im in imgs):
+            raise ValueError("images must be non-empty and finite")
         self.B = len(imgs)
         Ls = np.broadcast_to(np.asarray(L, dtype=np.int64), (self.B,)).copy()
         if np.any(Ls < 2) or np.any(Ls > 33):
@@ -95,6 +95,8 @@ class LiftBatch(object):
         self.kmax = max(self.K)
         self.lam = lam
         self.dtype = _hip.torch_dtype(precision)
+        _hip.require_device()
+        _bind()
         insts = (LiftInst * self.B)()
         off = offk = t0 = 0
         self.offs, self.offks, self.ntiles = [], [], []

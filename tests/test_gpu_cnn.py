@@ -234,3 +234,47 @@ def test_warm_start_dual_lies_in_each_instance_ball(gpu, fused):
         assert n.max() <= tvw[b] * (1 + 1e-6), (b, n.max(), tvw[b])
     # the small radius is active: the network's output exceeds it somewhere
     assert np.linalg.norm(ys[0].reshape(-1, 2), axis=1).max() > 0.9 * tvw[0]
+
+
+def test_warm_start_at_full_size_matches_oracle_windows(gpu):
+    """The bench's inference path at its real shape: the packed warm start of
+    two 1024x1024 instances (first and last layers reading and writing the
+    packed buffers, BF16 activations) against the oracle network on windows
+    of the same images.  Six 3x3 layers see 6 pixels around an output pixel,
+    so the oracle runs on a window grown by 6 pixels (clipped at the image
+    edge, where both sides pad with zeros) and its interior is compared.
+    Windows at a corner, an edge and the middle."""
+    import os
+    from opymize_amd.solvers.tv import TVBatch
+    from opymize_amd.tools import synth
+    from opymize_amd.warmstart.cnn import ConvNet, WarmStart
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "opymize_amd", "warmstart",
+                        "weights_tv_l2.npz")
+    net = ConvNet.load(path)
+    imgs = [synth.rects(1024, 1024, seed=80 + i) for i in range(2)]
+    lams = [2.0, 9.0]
+    bat = TVBatch(imgs, lams, "l2", precision="float", tile=(64, 6))
+    WarmStart(net, chunk=2).predict_into(bat, lams)
+    xs, ys = bat.solution()
+    params = [(net.W(l).cpu().numpy().reshape(9, 32, 32).astype(np.float64),
+               net.b(l).cpu().numpy().astype(np.float64)) for l in range(net.layers)]
+    m = net.layers
+    for b in range(2):
+        f = imgs[b].astype(np.float32)
+        X = xs[b].reshape(1024, 1024)
+        Y = ys[b].reshape(1024, 1024, 2)
+        for r0, c0 in ((0, 0), (0, 500), (480, 600), (1024 - 40, 1024 - 48)):
+            h, w = 40, 48
+            a0, a1 = max(0, r0 - m), min(1024, r0 + h + m)
+            b0, b1 = max(0, c0 - m), min(1024, c0 + w + m)
+            x = np.zeros((1, a1 - a0, b1 - b0, 32), dtype=np.float32)
+            x[0, ..., 0] = f[a0:a1, b0:b1] - np.float32(0.5)
+            x[0, ..., 1] = np.float32(np.log(np.float32(lams[b])) / 3.0)
+            out = cnn.net_forward(x, params)[-1][0]
+            win = out[r0 - a0:r0 - a0 + h, c0 - b0:c0 - b0 + w]
+            x_ref = f[r0:r0 + h, c0:c0 + w].astype(np.float64) + win[..., 0]
+            y_ref = win[..., 1:3]
+            nrm = np.linalg.norm(y_ref, axis=-1, keepdims=True)
+            y_ref = y_ref * np.minimum(1.0, 1.0 / np.maximum(nrm, 1e-300))  # |y| <= tvw = 1
+            rel_close(X[r0:r0 + h, c0:c0 + w], x_ref)
+            rel_close(Y[r0:r0 + h, c0:c0 + w], y_ref)

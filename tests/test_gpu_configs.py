@@ -136,8 +136,14 @@ def test_batched_uneven_512_certifies_every_instance(gpu):
     bat = TVBatch(imgs, lams, data, precision="float", tile=(64, 6), multi=4)
     best = autotune.tune(bat, steps="accel")
     print("uneven batch: tuned %s" % (best,), flush=True)
+    import time
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     res = bat.solve(steps="accel", step_bound=0.124, step_factor=4.0, step_factor_l1=1.0 / 256, max_iters=20000,
                     block_iters=48, tol=1e-5, certify=True)
+    torch.cuda.synchronize()
+    print("uneven batch: solved in %.2fs" % (time.perf_counter() - t0))
     assert all(np.isfinite(r["gap"]) for r in res)
     nl1 = sum(d == "l1" for d in data)
     cert = sum(r["status"] == 1 for r in res)
