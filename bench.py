@@ -70,7 +70,8 @@ def build_batch(args):
     log("generated %d images of %dx%d in %.1fs" % (len(imgs), args.size, args.size, time.time() - t))
     bat = TVBatch(imgs, [s["lam"] for s in specs], "l2", precision="float", tile=tuple(args.tile),
                   passes=args.passes,
-                  multi=args.multi if (args.step_rule != "adaptive" and args.passes == 1) else 1)
+                  multi=args.multi if (args.step_rule != "adaptive" and args.passes == 1) else 1,
+                  multi_kernel=getattr(args, "multi_kernel", "auto"), stream_rows=getattr(args, "stream_rows", 128))
     del imgs
     torch.cuda.synchronize()
     return bat, specs
@@ -93,7 +94,11 @@ def roofline_phase(bat, reps, rule):
     s = _hip.stream()
 
     def launch(name):
-        if name == "multi":
+        if name == "multi" and bat._use_stream(cfg):
+            bat._call("opy_tv_stream" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.f),
+                      P(bat.insts), P(bat.state), P(bat.active_inst), P(bat.unit_off), P(bat.nact),
+                      ctypes.byref(cfg), bat.multi, bat.stream_rows, bat.sgrid, s)
+        elif name == "multi":
             bat._call("opy_tv_multi" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.y[1]), P(bat.f),
                       P(bat.insts), P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact),
                       ctypes.byref(cfg), bat.multi, bat.grid, s)
@@ -109,7 +114,7 @@ def roofline_phase(bat, reps, rule):
             bat._call("opy_tv_primal" + sfx, P(bat.x[0]), P(bat.x[1]), P(bat.y[0]), P(bat.f), P(bat.insts),
                       P(bat.state), P(bat.active_inst), P(bat.act_off), P(bat.nact), ctypes.byref(cfg), bat.grid, s)
 
-    out = {"grid": bat.grid}
+    out = {"grid": bat.sgrid if (bat.multi > 1 and bat._use_stream(cfg)) else bat.grid}
     if bat.passes == 2:
         names = ("dual", "primal")
     else:
@@ -133,7 +138,7 @@ def roofline_phase(bat, reps, rule):
     return out
 
 
-def measured_traffic(npx, tile, passes, rule, multi):
+def measured_traffic(npx, tile, passes, rule, multi, stream=False):
     """(HBM bytes per full-batch launch of the dominant kernel, where they come
     from): the committed counter runs (FETCH_SIZE and WRITE_SIZE, each scaled
     by what it reads for a copy of known size), when they were taken on this
@@ -141,7 +146,7 @@ def measured_traffic(npx, tile, passes, rule, multi):
     if passes == 2:
         name = "tv_dual_kernel"
     elif multi > 1:
-        name = "tv_multi_kernel_k%d" % multi
+        name = ("tv_stream_kernel_k%d" if stream else "tv_multi_kernel_k%d") % multi
     else:
         name = "tv_iter_kernel"
     name += "_accel" if rule == "accel" else ""
@@ -203,7 +208,8 @@ def plain_same_rule(bat, specs, n, max_iters, args):
     for b in pick:
         H, W = bat.shapes[b]
         one = TVBatch([f_all[b].reshape(H, W)], [specs[b]["lam"]], "l2", precision="float", tile=tuple(args.tile),
-                      passes=args.passes, multi=bat.multi)
+                      passes=args.passes, multi=bat.multi, multi_kernel=bat.multi_kernel,
+                      stream_rows=bat.stream_rows)
         torch.cuda.synchronize()
         t = time.perf_counter()
         res = one.solve(steps=args.step_rule, step_bound=0.124, step_factor=args.step_factor, max_iters=max_iters,
@@ -500,6 +506,9 @@ def main():
     ap.add_argument("--multi", type=int, default=4, choices=[1, 2, 3, 4],
                     help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
+    ap.add_argument("--multi-kernel", default="auto", choices=["auto", "stream", "tile"],
+                    help="work split of the K-iteration launches (auto: streaming strips, tiles for precond steps)")
+    ap.add_argument("--stream-rows", type=int, default=128, help="rows per unit of the streaming kernel")
     ap.add_argument("--warm", action="store_true",
                     help="time the learned warm start (the network's prediction of x, y) instead of the cold start")
     ap.add_argument("--cold", action="store_true", help="start from x = y = 0 (the default)")
@@ -606,8 +615,13 @@ def main():
     elif "multi" in kt:
         k_launch = bat.multi
         kname, kms, kbytes = "multi", kt["multi"], MULTI_BYTES_PER_PX
-        kdesc = "tv_multi_kernel<float,1,%d,%d>, %s (%d iterations per launch on LDS-staged tiles)" % (
-            args.tile[1], k_launch, variant, k_launch)
+        if bat._use_stream(bat._cfg(args.step_rule)):
+            kdesc = ("tv_stream_kernel<float,1,%d>, %s (%d iterations per launch, one wavefront per %d x %d unit, "
+                     "the iterations pipelined down its rows)" % (k_launch, variant, k_launch, 64 - 2 * k_launch,
+                                                                  bat.stream_rows))
+        else:
+            kdesc = "tv_multi_kernel<float,1,%d,%d>, %s (%d iterations per launch on LDS-staged tiles)" % (
+                args.tile[1], k_launch, variant, k_launch)
     else:
         kname, kms, kbytes = "iter", kt["iter"], ITER_BYTES_PER_PX
         kdesc = "tv_iter_kernel<float,1,%d>, %s (one pass per iteration)" % (args.tile[1], variant)
@@ -644,7 +658,8 @@ def main():
                lift["roofline"]["achieved"]))
 
     from opymize_amd.solvers import tv as tvmod
-    traffic = measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch)
+    traffic = measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch,
+                               stream=k_launch > 1 and bat._use_stream(bat._cfg(args.step_rule)))
     line = {
         "metric": "solves/sec to rel PD-gap 1e-5: batched+warm-start vs plain PDHG; %HBM peak",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -664,6 +679,9 @@ def main():
                     "median_per_instance": float(np.median(nrestarts))},
                    "other_start": other,
                    "tile": list(args.tile), "block_iters": args.block_iters, "iterations_per_launch": k_launch,
+                   "multi_kernel": ("stream" if k_launch > 1 and bat._use_stream(bat._cfg(args.step_rule))
+                                    else "tile") if k_launch > 1 else None,
+                   "stream_rows": bat.stream_rows if k_launch > 1 else None,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
                    "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max()),
                    "plain_single_instance_solves_per_sec": plain["value"] if plain else None,

@@ -386,6 +386,25 @@ int opy_tv_multi_f64(const double* x_in, double* x_out, const double* y_in, doub
                      const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
                      const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int grid,
                      void* stream);
+/* the same k in {2,3,4} iterations in one pass over HBM with steps SCALAR or ACCEL, the same
+ * iterates bit for bit, by a different work split: one wavefront per unit of 64-2k image columns by
+ * `rows` image rows, walking down the unit and k rows above and below it with the k iterations
+ * pipelined one row apart (no LDS).  unit_off (B+1 entries) is the running instances' exclusive
+ * prefix of units, from opy_tv_stream_units with the same k and rows; `grid` as above (one block
+ * of 4 wavefronts per 4 units streams fastest).  PRECOND steps are refused (use opy_tv_multi).
+ * No reference counterpart beyond the iteration itself (PDHG.iteration_step, pdhg.py:124-153). */
+int opy_tv_stream_f32(const float* x_in, float* x_out, const float* y_in, float* y_out, const float* f,
+                      const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                      const int32_t* unit_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int rows,
+                      int grid, void* stream);
+int opy_tv_stream_f64(const double* x_in, double* x_out, const double* y_in, double* y_out, const double* f,
+                      const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
+                      const int32_t* unit_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int rows,
+                      int grid, void* stream);
+/* unit_off[a] = sum over the running instances a' < a (active_inst, nact as from opy_tv_compact) of
+ * ceil(W / (64 - 2k)) * ceil(H / rows); unit_off[nact] = the total.  One block. */
+int opy_tv_stream_units(const opy_tv_inst_t* insts, const int32_t* active_inst, const int32_t* nact,
+                        int32_t* unit_off, int k, int rows, void* stream);
 /* adaptive step rule per running instance from the residual partials (reference pdhg.py:155-183) */
 int opy_tv_adapt(const opy_tv_inst_t* insts, opy_adapt_state_t* steps, const opy_tv_check_t* checks,
                  const double* partials, int B, int C, const opy_adapt_params_t* params, void* stream);

@@ -143,6 +143,39 @@ __device__ inline double accel_theta(int steps, const opy_tv_inst_t& I, double t
   return (steps == OPY_TV_STEPS_ACCEL && I.data == OPY_TV_DATA_L2) ? 1.0 / sqrt(1.0 + 2.0 * I.lam * tau) : 1.0;
 }
 
+// Products whose rounded value is an iterate (x+ of the SSD prox, the
+// projected y+) are formed with contraction off, so that the compiler never
+// fuses them into a later addition: every fused kernel rounds the same values
+// the same way, also where several iterations share one basic block.
+template <typename T>
+__device__ inline T mul_rn(T a, T b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
+// |v|^2 of a vector of 2C components (the axis-0 ones first), in one fixed
+// order of roundings: per channel k the fused v_k^2 + round(v_{C+k}^2),
+// then the channels' sum
+template <typename T, int C>
+__device__ inline T sqnorm2(const T* v) {
+  T n2 = T(0);
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const T t = sizeof(T) == 4 ? (T)__builtin_fmaf((float)v[k], (float)v[k], (float)mul_rn(v[C + k], v[C + k]))
+                               : (T)__builtin_fma((double)v[k], (double)v[k], (double)mul_rn(v[C + k], v[C + k]));
+    n2 = k == 0 ? t : n2 + t;
+  }
+  return n2;
+}
+
+// x_bar = (1 + theta) x+ - theta x, as one fused multiply-add of theta1 =
+// 1 + theta and x+ with the rounded product theta x
+template <typename T>
+__device__ inline T overrelax(T theta1, T theta, T xp, T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(theta1, xp, -mul_rn(theta, x));
+  else return __builtin_fma(theta1, xp, -mul_rn(theta, x));
+}
+
 // x+ = prox_{tau G}(x - tau (A^T y)) for the data terms: SSD gives
 // (v + tl f) / (1 + tl) with tl = tau lam, formed as a * (v + b f) with
 // a = 1/(1 + tl) and b = tl rounded to T; L1 gives f + soft(v - f, tl).
@@ -180,7 +213,7 @@ struct PrimalStep {
       const T d = v - f;
       return f + (d > bk ? d - bk : (d < -bk ? d + bk : T(0)));
     } else {
-      return ak * (v + bk * f);
+      return mul_rn(ak, v + bk * f);
     }
   }
   __device__ T operator()(T x, T aty, T f, int r, int c, int H, int W, int k, const Cfg& cfg) const {
@@ -322,21 +355,18 @@ template <typename T, int C, bool PC, bool M = true>
 __device__ inline void yplus_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
                                 const DualStep<T, C, PC>& ds, T tvw, const Cfg& cfg, T* yp, T* g) {
   grad_px<T, C, M>(xb, xbd, xbr, r, c, H, W, cfg, g);
-  T n2 = T(0);
 #pragma unroll
-  for (int e = 0; e < 2 * C; ++e) {
-    yp[e] = y[e] + ds.template at<M>(e, r, c, H, W) * g[e];
-    n2 += yp[e] * yp[e];
-  }
+  for (int e = 0; e < 2 * C; ++e) yp[e] = y[e] + ds.template at<M>(e, r, c, H, W) * g[e];
+  const T n2 = sqnorm2<T, C>(yp);
   if constexpr (sizeof(T) == 4) {
     // branch-free: the scale is 1 (exact) inside the ball
     const T sc = n2 > tvw * tvw ? ball_scale(tvw, n2) : T(1);
 #pragma unroll
-    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+    for (int e = 0; e < 2 * C; ++e) yp[e] = mul_rn(yp[e], sc);
   } else if (n2 > tvw * tvw) {
     const T sc = ball_scale(tvw, n2);
 #pragma unroll
-    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+    for (int e = 0; e < 2 * C; ++e) yp[e] = mul_rn(yp[e], sc);
   }
 }
 
@@ -393,7 +423,7 @@ __device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int
     const T d = v - f;
     return f + (d > s.b ? d - s.b : (d < -s.b ? d + s.b : T(0)));
   } else {
-    return s.a * (v + s.b * f);
+    return mul_rn(s.a, v + s.b * f);
   }
 }
 
@@ -402,7 +432,6 @@ __device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int
 template <typename T, int C, bool M>
 __device__ inline void dual_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
                                const FoldedStep<T, C>& s, T tvw, T* yp) {
-  T n2 = T(0);
 #pragma unroll
   for (int k = 0; k < C; ++k) {
     T gd = xbd[k] - xb[k], gr = xbr[k] - xb[k];
@@ -410,18 +439,18 @@ __device__ inline void dual_px(const T* xb, const T* xbd, const T* xbr, const T*
     if (M && !(c < W - 1)) gr = T(0);
     yp[k] = y[k] + gd * s.q[0][k];
     yp[C + k] = y[C + k] + gr * s.q[1][k];
-    n2 += yp[k] * yp[k] + yp[C + k] * yp[C + k];
   }
+  const T n2 = sqnorm2<T, C>(yp);
   if constexpr (sizeof(T) == 4) {
     // min(1, tvw / |v|): 1 inside the ball (and for v = 0, where the
     // reciprocal square root is +inf)
     const T sc = __builtin_fminf(T(1), ball_scale(tvw, n2));
 #pragma unroll
-    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+    for (int e = 0; e < 2 * C; ++e) yp[e] = mul_rn(yp[e], sc);
   } else if (n2 > tvw * tvw) {
     const T sc = ball_scale(tvw, n2);
 #pragma unroll
-    for (int e = 0; e < 2 * C; ++e) yp[e] *= sc;
+    for (int e = 0; e < 2 * C; ++e) yp[e] = mul_rn(yp[e], sc);
   }
 }
 

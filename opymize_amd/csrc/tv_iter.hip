@@ -171,13 +171,13 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
         for (int k = 0; k < C; ++k) {
           const T p = kFold ? fold_prox(xrh[k], frh[k], yo, yrh, yl1, rx, cr, k)
                             : pstep(xrh[k], ah[k], frh[k], rx, cr, k);
-          xbh[k] = (T(1) + theta) * p - theta * xrh[k];
+          xbh[k] = overrelax(T(1) + theta, theta, p, xrh[k]);
           xdh[k] = xrh[k] - p;
         }
       }
       // xbar = 2x+ - x and xd = x - x+ (and their right neighbours) are formed
       // where they are used, which keeps fewer values live
-      auto xbar = [&](int j, int k) { return (T(1) + theta) * xp[j][k] - theta * xv[j][k]; };
+      auto xbar = [&](int j, int k) { return overrelax(T(1) + theta, theta, xp[j][k], xv[j][k]); };
       // ---- left halo (adaptive), transposed: lane k forms y+ at (y+ row k, cl), t=1 part
       T yplh[C];
 #pragma unroll
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kTB) tv_iter_kernel(const T* __restrict__ xin,
 #pragma unroll
         for (int k = 0; k < C; ++k) {
           const T p = pstep(xlh[k], ah[k], flh[k], rx, cl, k);
-          xbl[k] = (T(1) + theta) * p - theta * xlh[k];
+          xbl[k] = overrelax(T(1) + theta, theta, p, xlh[k]);
         }
 #pragma unroll
         for (int k = 0; k < C; ++k) {

@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kTB, 3) tv_multi_kernel(const T* __restrict__ 
             const T p = PC ? prox_pc.template apply<L1, M>(xv[j][k], aty[k], fv[j][k], r, c, H, W, k, cfg)
                            : primal_px<T, C, L1, M>(xv[j][k], fv[j][k], yv[j][k], yv[j][C + k], yup0[k], yl1[k], r,
                                                     c, H, W, k, fs);
-            sxb[k][q] = (T(1) + theta) * p - theta * xv[j][k];
+            sxb[k][q] = overrelax(T(1) + theta, theta, p, xv[j][k]);
             xv[j][k] = p;
           }
         }

@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
       for (int k = 0; k < C; ++k) {
         const T a = ax[j][k], b = bx[j][k];
         ax[j][k] = a - b;
-        bx[j][k] = (T(1) + theta) * b - theta * a;
+        bx[j][k] = overrelax(T(1) + theta, theta, b, a);
       }
 #pragma unroll
     for (int j = j0; j <= RPT; ++j)
@@ -194,11 +194,11 @@ __global__ void __launch_bounds__(kTB) tv_dual_kernel(const T* __restrict__ xo, 
         const T a = axr[j][k], b = bxr[j][k];
         const T sd = shfl_down1(ax[j][k]), sb = shfl_down1(bx[j][k]);
         axr[j][k] = rhalo ? a - b : sd;
-        bxr[j][k] = rhalo ? (T(1) + theta) * b - theta * a : sb;
+        bxr[j][k] = rhalo ? overrelax(T(1) + theta, theta, b, a) : sb;
         if (kAdapt) {
           const T al = axl[j][k], bl = bxl[j][k];
           axl[j][k] = al - bl;
-          bxl[j][k] = (T(1) + theta) * bl - theta * al;
+          bxl[j][k] = overrelax(T(1) + theta, theta, bl, al);
         }
       }
     double s[4] = {0.0, 0.0, 0.0, 0.0};

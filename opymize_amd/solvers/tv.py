@@ -69,6 +69,7 @@ _SIGS = {
     "opy_tv_dual": [ctypes.c_void_p] * 10 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_iter": [ctypes.c_void_p] * 11 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_multi": [ctypes.c_void_p] * 11 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+    "opy_tv_stream": [ctypes.c_void_p] * 11 + [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_gap": [ctypes.c_void_p] * 9 + [ctypes.c_int, ctypes.c_void_p],
     "opy_tv_reset": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4,
 }
@@ -82,6 +83,7 @@ _PLAIN = {
                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "opy_tv_compact": [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int,
                                                                                       ctypes.c_void_p],
+    "opy_tv_stream_units": [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
 }
 _bound = False
 
@@ -125,10 +127,17 @@ class TVBatch(object):
             6 or 8), 1 one iteration per launch; adaptive steps always take
             one iteration per launch (their rule needs every iteration's
             residuals)
+    multi_kernel: how K > 1 iterations per launch are split over the GPU:
+            "stream" (one wavefront per strip of 64 - 2K columns by
+            stream_rows rows, the K iterations pipelined down the rows), "tile"
+            (LDS-staged tiles), or "auto" (stream, except for preconditioned
+            steps, which only the tile kernel runs); both give the iterates of
+            K single launches bit for bit
+    stream_rows: rows per unit of the streaming kernel
     """
 
     def __init__(self, images, lam, data="l2", tvw=1.0, precision="float", tile=(64, 4), h=(1.0, 1.0),
-                 weights=None, grid=None, passes=1, multi=1):
+                 weights=None, grid=None, passes=1, multi=1, multi_kernel="auto", stream_rows=128):
         _hip.require_device()
         _bind()
         self.B = len(images)
@@ -167,7 +176,14 @@ class TVBatch(object):
         if passes not in (1, 2):
             raise ValueError("passes must be 1 or 2")
         self.passes = passes
+        if multi_kernel not in ("auto", "stream", "tile"):
+            raise ValueError("multi_kernel must be 'auto', 'stream' or 'tile'")
+        self.multi_kernel = multi_kernel
+        self.unit_off = torch.zeros(self.B + 1, dtype=torch.int32, device="cuda")
+        self.stream_rows = 128
+        self.total_units, self.sgrid = 0, 0
         self.set_multi(multi)
+        self.set_stream_rows(stream_rows)
         self.set_tile(*tile)
         self.graph = None
         self.cur = 0  # buffer holding the current iterate
@@ -180,6 +196,53 @@ class TVBatch(object):
         if int(k) > 1 and self.passes != 1:
             raise ValueError("multi > 1 needs passes=1")
         self.multi = int(k)
+        self._stream_layout()
+
+    def set_stream_rows(self, rows):
+        """Rows per unit of the streaming multi-iteration kernel."""
+        if not 1 <= int(rows) <= 1 << 20:
+            raise ValueError("stream_rows must be in [1, 2^20]")
+        self.stream_rows = int(rows)
+        self._stream_layout()
+
+    def _stream_layout(self):
+        """Units of the streaming kernel over the whole batch (sizes its
+        launches) and the running instances' unit prefix."""
+        if self.multi < 2:
+            self.total_units = 0
+            return
+        sw, rows = 64 - 2 * self.multi, self.stream_rows
+        self.total_units = sum((-(-W // sw)) * (-(-H // rows)) for (H, W) in self.shapes)
+        if self._grid_override is None:
+            self.sgrid = self._stream_grid(self.total_units)
+        else:
+            self.sgrid = int(self._grid_override)
+        if hasattr(self, "act_off"):
+            self._stream_units()
+
+    def _stream_units(self):
+        if self.multi > 1:
+            P = _hip.ptr
+            self._call("opy_tv_stream_units", P(self.insts), P(self.active_inst), P(self.nact), P(self.unit_off),
+                       self.multi, self.stream_rows, _hip.stream())
+
+    def _stream_grid(self, units):
+        """Blocks (4 wavefronts, one unit each) for `units` running units,
+        bucketed like _launch_grid and rounded up to a multiple of the 8
+        XCDs; blocks past the running units return at once."""
+        full = max(1, -(-self.total_units // 4))
+        q = max(256, -(-full // 16))
+        blocks = max(1, min(full, -(-max(-(-units // 4), 1) // q) * q))
+        return -(-blocks // 8) * 8
+
+    def _use_stream(self, cfg):
+        if self.multi_kernel == "tile" or self.multi < 2:
+            return False
+        if cfg.steps == STEP_MODES["precond"]:
+            if self.multi_kernel == "stream":
+                raise ValueError("the streaming kernel has no preconditioned steps; use multi_kernel='tile'")
+            return False
+        return True
 
     def set_tile(self, tw, rpt):
         if self.passes == 1:
@@ -240,6 +303,9 @@ class TVBatch(object):
         nrun = int(self.nact.item())
         tiles = int(self.act_off[nrun].item()) if nrun > 0 else 0
         self.grid = self._launch_grid(tiles)
+        if self.multi > 1 and self._grid_override is None:
+            units = int(self.unit_off[nrun].item()) if nrun > 0 else 0
+            self.sgrid = self._stream_grid(units)
         return nrun
 
     def _cfg(self, steps):
@@ -347,14 +413,20 @@ class TVBatch(object):
         P = _hip.ptr
         self._call("opy_tv_compact", P(self.insts), P(self.checks), self.B, P(self.active_inst), P(self.act_off),
                    P(self.nact), P(self.iter_counter), int(iters_in_block), _hip.stream())
+        self._stream_units()
 
     def _multi_launch(self, p, cfg, k):
         """k iterations in one launch from buffer p, then the k step updates."""
         P = _hip.ptr
         s = _hip.stream()
-        self._call("opy_tv_multi" + self._sfx(), P(self.x[p]), P(self.x[1 - p]), P(self.y[p]), P(self.y[1 - p]),
-                   P(self.f), P(self.insts), P(self.state), P(self.active_inst), P(self.act_off), P(self.nact),
-                   ctypes.byref(cfg), int(k), self.grid, s)
+        if self._use_stream(cfg) and k == self.multi:
+            self._call("opy_tv_stream" + self._sfx(), P(self.x[p]), P(self.x[1 - p]), P(self.y[p]),
+                       P(self.y[1 - p]), P(self.f), P(self.insts), P(self.state), P(self.active_inst),
+                       P(self.unit_off), P(self.nact), ctypes.byref(cfg), int(k), self.stream_rows, self.sgrid, s)
+        else:
+            self._call("opy_tv_multi" + self._sfx(), P(self.x[p]), P(self.x[1 - p]), P(self.y[p]),
+                       P(self.y[1 - p]), P(self.f), P(self.insts), P(self.state), P(self.active_inst),
+                       P(self.act_off), P(self.nact), ctypes.byref(cfg), int(k), self.grid, s)
         if cfg.steps == STEP_MODES["accel"]:
             self._call("opy_tv_accel_n", P(self.insts), P(self.state), P(self.checks), self.B, int(k), s)
 
@@ -384,7 +456,7 @@ class TVBatch(object):
         block of a solve runs eagerly, which loads its kernels before any
         capture; a new key is captured (that records without running) and
         then replayed."""
-        key = (self.grid, self.cur)
+        key = (self.grid, self.sgrid, self.cur)
         if key in cache:
             g, end = cache[key]
             g.replay()

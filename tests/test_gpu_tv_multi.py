@@ -1,6 +1,7 @@
-"""Several fused TV iterations per launch (opy_tv_multi, TVBatch(multi=K)):
-the iterates of K iterations on LDS-staged tiles against the oracle and
-against K single-iteration launches.  Tolerances as in test_gpu_tv_batch.py
+"""Several fused TV iterations per launch (opy_tv_multi, TVBatch(multi=K,
+multi_kernel="tile")): the iterates of K iterations on LDS-staged tiles
+against the oracle and against K single-iteration launches (the streaming
+kernel has its own tests in test_gpu_tv_stream.py).  Tolerances as in test_gpu_tv_batch.py
 (FP64 1e-9, FP32 5e-4 max abs against the oracle); against the single-launch
 path 1e-6 (FP32) and 1e-12 (FP64): the per-pixel arithmetic is the same, so
 only the compiler's instruction selection can differ."""
@@ -23,7 +24,7 @@ def lds_bytes(precision, C, rpt, k):
     return rows * (64 + 2 * k + 2) * 3 * C * size + k * (3 + 4 * C) * size
 
 
-def _batch(precision, tile, multi, shapes=SHAPES, C=1, seed=0, data=None):
+def _batch(precision, tile, multi, shapes=SHAPES, C=1, seed=0, data=None, kernel="tile"):
     from opymize_amd.solvers.tv import TVBatch
     rng = np.random.default_rng(seed)
     if C == 1:
@@ -33,7 +34,8 @@ def _batch(precision, tile, multi, shapes=SHAPES, C=1, seed=0, data=None):
     lams = list(np.round(np.exp(rng.uniform(0.0, np.log(20.0), len(shapes))), 3))
     if data is None:
         data = ["l2" if i % 3 != 1 else "l1" for i in range(len(shapes))]
-    return TVBatch(imgs, lams, data, precision=precision, tile=tile, multi=multi), imgs, lams, data
+    return (TVBatch(imgs, lams, data, precision=precision, tile=tile, multi=multi, multi_kernel=kernel), imgs, lams,
+            data)
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
@@ -218,6 +220,7 @@ def test_multi_rejects_bad_arguments(gpu):
     with pytest.raises(ValueError):
         TVBatch([np.zeros((8, 8))], 1.0, passes=2, multi=2)
     bat.set_multi(2)  # rows per thread 12 is a single-pass tile, not a multi-iteration one
+    bat.multi_kernel = "tile"  # (the streaming kernel has no tiles)
     with pytest.raises(_hip.HipError, match="rows per thread"):
         bat.solve(steps="const", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30)
     for bad in ((1.0, 96), (0.5, -1), (0.5,)):
