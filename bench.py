@@ -27,11 +27,15 @@ Printed next to it:
     instances spread over the lambda range of the same batch, and the same
     instances one at a time through the batched solver's own kernels and
     step rule (so that ratio isolates the gain of batching);
-  * the roofline of the dominant kernel (by default the kernel that runs K
-    iterations per launch on LDS-staged tiles; the single-pass iteration
-    kernel with --multi 1 or adaptive steps; the dual pass with --passes 2):
-    algorithmic bytes of a full-batch launch over the average duration of
-    eager full-batch launches, timed with HIP events on the launch stream;
+  * the roofline of the dominant kernel (by default the streaming kernel
+    that runs K = 6 iterations per launch, one wavefront per strip of 52
+    columns by 128 rows, the iterations pipelined down the rows; --multi-kernel
+    tile: the LDS-tile kernel, K <= 4; the single-pass iteration kernel with
+    --multi 1 or adaptive steps; the dual pass with --passes 2): algorithmic
+    bytes of a full-batch launch over the average duration of eager
+    full-batch launches, timed with HIP events on the launch stream; with
+    K > 4 also the same kernel at K = 4 (roofline_k4), where HBM rather than
+    instruction issue bounds it;
   * the CPU oracle (float64 numpy/scipy, one thread) on a bounded sample,
     and a pool of processes running it on as many instances at once.
 
@@ -129,6 +133,24 @@ def roofline_phase(bat, reps, rule):
         e1.record(stream)
         torch.cuda.synchronize()
         out[name] = e0.elapsed_time(e1) / reps
+    if names == ("multi",) and bat._use_stream(cfg) and bat.multi != 4:
+        # the same kernel at 4 iterations per launch, where HBM rather than
+        # instruction issue bounds it
+        k0 = bat.multi
+        bat.set_multi(4)
+        bat.refresh_grid()
+        launch("multi")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch("multi")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out["multi_k4"] = e0.elapsed_time(e1) / reps
+        out["grid_k4"] = bat.sgrid
+        bat.set_multi(k0)
+        bat.refresh_grid()
     # copy of a known size (2^28 floats = 1 GiB read + 1 GiB written): the counter scale
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
@@ -503,7 +525,7 @@ def main():
     ap.add_argument("--max-iters", type=int, default=10000)
     ap.add_argument("--block-iters", type=int, default=48)
     ap.add_argument("--tile", type=int, nargs=2, default=[64, 6])
-    ap.add_argument("--multi", type=int, default=4, choices=[1, 2, 3, 4],
+    ap.add_argument("--multi", type=int, default=6, choices=[1, 2, 3, 4, 5, 6, 7, 8],
                     help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
     ap.add_argument("--multi-kernel", default="auto", choices=["auto", "stream", "tile"],
@@ -607,6 +629,8 @@ def main():
         log("batched (%s start, for comparison): %d/%d certified in %.2fs" % (other["start"], c, args.batch, dt))
 
     kt = roofline_phase(bat, args.reps, args.step_rule)
+    stream_used = bat.multi > 1 and bat._use_stream(bat._cfg(args.step_rule))
+    stream_rows = bat.stream_rows
     variant = "adaptive" if args.step_rule == "adaptive" else "accelerated steps"
     k_launch = 1
     if bat.passes == 2:
@@ -615,10 +639,10 @@ def main():
     elif "multi" in kt:
         k_launch = bat.multi
         kname, kms, kbytes = "multi", kt["multi"], MULTI_BYTES_PER_PX
-        if bat._use_stream(bat._cfg(args.step_rule)):
+        if stream_used:
             kdesc = ("tv_stream_kernel<float,1,%d>, %s (%d iterations per launch, one wavefront per %d x %d unit, "
                      "the iterations pipelined down its rows)" % (k_launch, variant, k_launch, 64 - 2 * k_launch,
-                                                                  bat.stream_rows))
+                                                                  stream_rows))
         else:
             kdesc = "tv_multi_kernel<float,1,%d,%d>, %s (%d iterations per launch on LDS-staged tiles)" % (
                 args.tile[1], k_launch, variant, k_launch)
@@ -658,8 +682,7 @@ def main():
                lift["roofline"]["achieved"]))
 
     from opymize_amd.solvers import tv as tvmod
-    traffic = measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch,
-                               stream=k_launch > 1 and bat._use_stream(bat._cfg(args.step_rule)))
+    traffic = measured_traffic(npx, args.tile, args.passes, args.step_rule, k_launch, stream=stream_used)
     line = {
         "metric": "solves/sec to rel PD-gap 1e-5: batched+warm-start vs plain PDHG; %HBM peak",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -679,9 +702,8 @@ def main():
                     "median_per_instance": float(np.median(nrestarts))},
                    "other_start": other,
                    "tile": list(args.tile), "block_iters": args.block_iters, "iterations_per_launch": k_launch,
-                   "multi_kernel": ("stream" if k_launch > 1 and bat._use_stream(bat._cfg(args.step_rule))
-                                    else "tile") if k_launch > 1 else None,
-                   "stream_rows": bat.stream_rows if k_launch > 1 else None,
+                   "multi_kernel": ("stream" if stream_used else "tile") if k_launch > 1 else None,
+                   "stream_rows": stream_rows if stream_used else None,
                    "certified": int(certified), "instances": int(args.steps * args.batch),
                    "iterations_median": float(np.median(iters)), "iterations_max": int(iters.max()),
                    "plain_single_instance_solves_per_sec": plain["value"] if plain else None,
@@ -698,10 +720,24 @@ def main():
                      "traffic": traffic[0], "traffic_source": traffic[1],
                      "kernel": "%s: %d B/pixel algorithmic x %d pixels per launch" % (kdesc, kbytes, npx),
                      "avg_ms": kms, "ms_per_iteration": kms / k_launch,
-                     "launch_ms": {k: v for k, v in kt.items() if k != "grid"},
+                     "launch_ms": {k: v for k, v in kt.items() if not k.startswith("grid")},
                      "blocks_per_launch": kt["grid"]},
         "cpu_baseline": cpu,
     }
+    if "multi_k4" in kt:
+        # the same kernel at 4 iterations per launch (same bytes per launch):
+        # beyond 4 the launch is bound by instruction issue, not HBM, and it is
+        # the time per iteration the solve needs
+        a4 = kbytes * npx / (kt["multi_k4"] * 1e-3) / 1e9
+        line["roofline"]["note"] = ("%d iterations per launch: bound by instruction issue (VALU), %.3f ms per "
+                                    "iteration; the same kernel at 4 per launch is HBM-bound, see roofline_k4"
+                                    % (k_launch, kms / k_launch))
+        line["roofline_k4"] = {"bound": "hbm", "achieved": a4, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": a4 / HBM_PEAK_GBS, "avg_ms": kt["multi_k4"],
+                               "ms_per_iteration": kt["multi_k4"] / 4, "blocks_per_launch": kt["grid_k4"],
+                               "kernel": "the same streaming kernel with 4 iterations per launch (%d x %d units), "
+                                         "%d B/pixel algorithmic x %d pixels per launch"
+                                         % (56, stream_rows, kbytes, npx)}
     print(json.dumps(line), flush=True)
 
 

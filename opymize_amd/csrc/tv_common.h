@@ -153,6 +153,15 @@ __device__ inline T mul_rn(T a, T b) {
   return a * b;
 }
 
+// soft threshold d - clamp(d, -b, b) (b >= 0): d - b above b, d + b below
+// -b, +0 between, without branches (for finite d the same values as the
+// three-way select)
+template <typename T>
+__device__ inline T soft_rest(T d, T b) {
+  if constexpr (sizeof(T) == 4) return d - __builtin_fminf(__builtin_fmaxf(d, -b), b);
+  else return d - __builtin_fmin(__builtin_fmax(d, -b), b);
+}
+
 // |v|^2 of a vector of 2C components (the axis-0 ones first), in one fixed
 // order of roundings: per channel k the fused v_k^2 + round(v_{C+k}^2),
 // then the channels' sum
@@ -211,7 +220,7 @@ struct PrimalStep {
     const T v = x - t * aty;
     if constexpr (L1) {
       const T d = v - f;
-      return f + (d > bk ? d - bk : (d < -bk ? d + bk : T(0)));
+      return f + soft_rest(d, bk);
     } else {
       return mul_rn(ak, v + bk * f);
     }
@@ -421,7 +430,7 @@ __device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int
   const T v = x + (y0 - yup0) * s.p[0][k] + (y1 - yleft1) * s.p[1][k];
   if constexpr (L1) {
     const T d = v - f;
-    return f + (d > s.b ? d - s.b : (d < -s.b ? d + s.b : T(0)));
+    return f + soft_rest(d, s.b);
   } else {
     return mul_rn(s.a, v + s.b * f);
   }

@@ -265,6 +265,7 @@ void launch_stream(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy
 }
 
 constexpr int kStreamMaxRows = 1 << 20;
+constexpr int kStreamMaxK = 8;  // iterations per launch (k > 4 for C <= 2: registers)
 
 template <typename T>
 int stream_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv_inst_t* insts,
@@ -273,7 +274,8 @@ int stream_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv
   if (int rc = check_cfg(cfg)) return rc;
   if (cfg->steps == OPY_TV_STEPS_PRECOND)
     return fail(kInvalid, "tv_stream: preconditioned steps vary per pixel; use opy_tv_multi");
-  if (k != 2 && k != 3 && k != 4) return fail(kInvalid, "tv_stream: k = %d not in {2,3,4}", k);
+  if (k < 2 || k > kStreamMaxK) return fail(kInvalid, "tv_stream: k = %d not in [2, %d]", k, kStreamMaxK);
+  if (k > 4 && cfg->C > 2) return fail(kInvalid, "tv_stream: k = %d > 4 needs C <= 2 (got C = %d)", k, cfg->C);
   if (rows < 1 || rows > kStreamMaxRows) return fail(kInvalid, "tv_stream: rows = %d not in [1, %d]", rows, kStreamMaxRows);
   if (!xi || !xo || !yi || !yo || !f || !insts || !st || !ai || !uo || !na || grid < 0)
     return fail(kInvalid, "tv_stream: null pointer or negative grid");
@@ -287,12 +289,23 @@ int stream_impl(const T* xi, T* xo, const T* yi, T* yo, const T* f, const opy_tv
     case 3: OPY_S(CC, 3); break;    \
     default: OPY_S(CC, 4); break;   \
   }
+#define OPY_SK8(CC)                 \
+  switch (k) {                      \
+    case 2: OPY_S(CC, 2); break;    \
+    case 3: OPY_S(CC, 3); break;    \
+    case 4: OPY_S(CC, 4); break;    \
+    case 5: OPY_S(CC, 5); break;    \
+    case 6: OPY_S(CC, 6); break;    \
+    case 7: OPY_S(CC, 7); break;    \
+    default: OPY_S(CC, 8); break;   \
+  }
   switch (cfg->C) {
-    case 1: OPY_SK(1); break;
-    case 2: OPY_SK(2); break;
+    case 1: OPY_SK8(1); break;
+    case 2: OPY_SK8(2); break;
     case 3: OPY_SK(3); break;
     default: OPY_SK(4); break;
   }
+#undef OPY_SK8
 #undef OPY_SK
 #undef OPY_S
   return check_launch("tv_stream");
@@ -306,7 +319,7 @@ using namespace opy;
 OPY_API int opy_tv_stream_units(const opy_tv_inst_t* insts, const int32_t* active_inst, const int32_t* nact,
                                 int32_t* unit_off, int k, int rows, void* s) {
   if (!insts || !active_inst || !nact || !unit_off) return fail(kInvalid, "tv_stream_units: null pointer");
-  if (k != 2 && k != 3 && k != 4) return fail(kInvalid, "tv_stream_units: k = %d not in {2,3,4}", k);
+  if (k < 2 || k > kStreamMaxK) return fail(kInvalid, "tv_stream_units: k = %d not in [2, %d]", k, kStreamMaxK);
   if (rows < 1 || rows > kStreamMaxRows)
     return fail(kInvalid, "tv_stream_units: rows = %d not in [1, %d]", rows, kStreamMaxRows);
   hipLaunchKernelGGL(tv_stream_units_kernel, dim3(1), dim3(kTB), 0, as_stream(s), insts, active_inst, nact, unit_off,

@@ -123,10 +123,11 @@ class TVBatch(object):
     tile:   (tile width, rows per thread); with passes=1 the width is 64
     multi:  iterations per pass over HBM with the constant, preconditioned
             and accelerated step rules (passes=1): K in {2, 3, 4} runs K
-            iterations per launch on LDS-staged tiles (rows per thread 2, 4,
-            6 or 8), 1 one iteration per launch; adaptive steps always take
-            one iteration per launch (their rule needs every iteration's
-            residuals)
+            iterations per launch (on LDS-staged tiles of rows per thread 2,
+            4, 6 or 8, or streamed, see multi_kernel), K in 5..8 on the
+            streaming kernel only (C <= 2, no preconditioned steps), 1 one
+            iteration per launch; adaptive steps always take one iteration
+            per launch (their rule needs every iteration's residuals)
     multi_kernel: how K > 1 iterations per launch are split over the GPU:
             "stream" (one wavefront per strip of 64 - 2K columns by
             stream_rows rows, the K iterations pipelined down the rows), "tile"
@@ -191,8 +192,10 @@ class TVBatch(object):
 
     # -- layout --------------------------------------------------------------
     def set_multi(self, k):
-        if int(k) not in (1, 2, 3, 4):
-            raise ValueError("multi must be 1, 2, 3 or 4 iterations per launch")
+        if not 1 <= int(k) <= 8:
+            raise ValueError("multi must be 1 to 8 iterations per launch")
+        if int(k) > 4 and (self.multi_kernel == "tile" or self.C > 2):
+            raise ValueError("multi > 4 runs on the streaming kernel only, for C <= 2")
         if int(k) > 1 and self.passes != 1:
             raise ValueError("multi > 1 needs passes=1")
         self.multi = int(k)
@@ -239,8 +242,9 @@ class TVBatch(object):
         if self.multi_kernel == "tile" or self.multi < 2:
             return False
         if cfg.steps == STEP_MODES["precond"]:
-            if self.multi_kernel == "stream":
-                raise ValueError("the streaming kernel has no preconditioned steps; use multi_kernel='tile'")
+            if self.multi_kernel == "stream" or self.multi > 4:
+                raise ValueError("the streaming kernel has no preconditioned steps; use multi_kernel='tile' "
+                                 "(at most 4 iterations per launch)")
             return False
         return True
 

@@ -216,11 +216,13 @@ def test_multi_rejects_bad_arguments(gpu):
     from opymize_amd.solvers.tv import TVBatch
     bat, *_ = make_batch("float", (64, 12))
     with pytest.raises(ValueError):
-        bat.set_multi(5)
+        bat.set_multi(9)
+    bat.multi_kernel = "tile"
+    with pytest.raises(ValueError, match="streaming kernel only"):
+        bat.set_multi(5)  # the tile kernel runs at most 4 iterations per launch
     with pytest.raises(ValueError):
         TVBatch([np.zeros((8, 8))], 1.0, passes=2, multi=2)
-    bat.set_multi(2)  # rows per thread 12 is a single-pass tile, not a multi-iteration one
-    bat.multi_kernel = "tile"  # (the streaming kernel has no tiles)
+    bat.set_multi(2)  # rows per thread 12 is a single-pass tile, not a multi-iteration one (of the tile kernel)
     with pytest.raises(_hip.HipError, match="rows per thread"):
         bat.solve(steps="const", step_bound=0.124, max_iters=2, block_iters=2, tol=1e-30)
     for bad in ((1.0, 96), (0.5, -1), (0.5,)):

@@ -58,10 +58,11 @@ def test_stream_equals_tile_kernel(gpu, precision, multi, steps, rows):
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
-@pytest.mark.parametrize("multi", [2, 4])
+@pytest.mark.parametrize("multi", [2, 4, 5, 6, 8])
 def test_stream_equals_single_launches(gpu, precision, multi):
     """K iterations per streaming launch against K single-iteration launches
-    (multi = 1), accelerated steps: the same bits."""
+    (multi = 1), accelerated steps: the same bits (K = 5: two launches and two
+    single iterations per block of 12)."""
     out = {}
     for m in (1, multi):
         bat, *_ = _batch(precision, m, "stream", seed=11)
@@ -196,9 +197,36 @@ def test_stream_full_size_launch(gpu):
     assert float(outs[0][0].abs().max()) > 0
 
 
+@pytest.mark.parametrize("multi", [6, 8])
+def test_stream_many_iterations_match_oracle(gpu, multi):
+    """Beyond the tile kernel's 4 iterations per launch: FP64 iterates of the
+    oracle after 96 iterations (accelerated, restarts), C = 1 and C = 2."""
+    for C, shapes in ((1, SHAPES), (2, [(33, 150), (9, 300), (70, 41)])):
+        bat, imgs, lams, data = _batch("double", multi, "stream", shapes=shapes, C=C, rows=24)
+        k = 96
+        bat.solve(steps="accel", step_bound=0.124, max_iters=k, block_iters=48, tol=1e-30, restart=(0.3, 48))
+        xs, ys = bat.solution()
+        for b, im in enumerate(imgs):
+            H, W = im.shape[:2]
+            prob = opd.tv_problem(im.reshape(-1, C), (H, W), lams[b], data[b])
+            xo, yo, _, _ = opd.solve(prob, steps="accel", step_bound=0.124, term_maxiter=k, granularity=48,
+                                     term_pd_gap=1e-30, restart=(0.3, 48))
+            assert np.abs(xs[b] - xo).max() < 1e-9, (C, b)
+            assert np.abs(ys[b] - yo).max() < 1e-9, (C, b)
+
+
 def test_stream_rejects_bad_arguments(gpu):
     from opymize_amd import _hip
     from opymize_amd.solvers.tv import TVBatch
+    with pytest.raises(ValueError, match="streaming kernel only"):
+        TVBatch([np.zeros((8, 8))], 1.0, multi=6, multi_kernel="tile")
+    with pytest.raises(ValueError, match="streaming kernel only"):
+        TVBatch([np.zeros((8, 8, 3))], 1.0, multi=6)
+    with pytest.raises(ValueError):
+        TVBatch([np.zeros((8, 8))], 1.0, multi=9)
+    six = TVBatch([np.zeros((8, 8))], 1.0, multi=6)
+    with pytest.raises(ValueError, match="preconditioned"):
+        six.solve(steps="precond", step_bound=0.124, max_iters=12, block_iters=12, tol=1e-30)
     with pytest.raises(ValueError):
         TVBatch([np.zeros((8, 8))], 1.0, multi=2, multi_kernel="lds")
     with pytest.raises(ValueError):
@@ -216,8 +244,12 @@ def test_stream_rejects_bad_arguments(gpu):
 
     with pytest.raises(_hip.HipError, match="preconditioned"):
         call(bat._cfg("precond"), 4, 128)
-    with pytest.raises(_hip.HipError, match="k = 5"):
-        call(bat._cfg("accel"), 5, 128)
+    with pytest.raises(_hip.HipError, match="k = 9"):
+        call(bat._cfg("accel"), 9, 128)
+    c3 = bat._cfg("accel")
+    c3.C = 3
+    with pytest.raises(_hip.HipError, match="C <= 2"):
+        call(c3, 6, 128)
     with pytest.raises(_hip.HipError, match="rows"):
         call(bat._cfg("accel"), 4, 0)
     with pytest.raises(_hip.HipError, match="rows"):
