@@ -230,7 +230,7 @@ def plain_same_rule(bat, specs, n, max_iters, args):
     for b in pick:
         H, W = bat.shapes[b]
         one = TVBatch([f_all[b].reshape(H, W)], [specs[b]["lam"]], "l2", precision="float", tile=tuple(args.tile),
-                      passes=args.passes, multi=bat.multi, multi_kernel=bat.multi_kernel,
+                      passes=args.passes, multi=min(bat.multi, 4), multi_kernel=bat.multi_kernel,
                       stream_rows=bat.stream_rows)
         torch.cuda.synchronize()
         t = time.perf_counter()

@@ -61,6 +61,7 @@ RESTART_DEFAULT = (0.5, 96)
 CHECK_DOUBLES = ctypes.sizeof(TVCheck) // 8
 RESTART_DOUBLES = ctypes.sizeof(TVRestart) // 8
 STATE_DOUBLES = _hip.ADAPT_STATE_DOUBLES
+STREAM_MIN_UNITS = 4096
 assert ctypes.sizeof(TVInst) == 48 and ctypes.sizeof(TVCfg) == 64 and ctypes.sizeof(TVCheck) == 72
 assert ctypes.sizeof(TVRestart) == 24
 
@@ -131,9 +132,11 @@ class TVBatch(object):
     multi_kernel: how K > 1 iterations per launch are split over the GPU:
             "stream" (one wavefront per strip of 64 - 2K columns by
             stream_rows rows, the K iterations pipelined down the rows), "tile"
-            (LDS-staged tiles), or "auto" (stream, except for preconditioned
-            steps, which only the tile kernel runs); both give the iterates of
-            K single launches bit for bit
+            (LDS-staged tiles, K <= 4), or "auto": stream while the running
+            instances give at least stream_min_units units (decided at each
+            check), else tiles with min(K, 4) iterations per launch, and
+            tiles for preconditioned steps; every choice gives the iterates
+            of single launches bit for bit
     stream_rows: rows per unit of the streaming kernel
     """
 
@@ -183,6 +186,10 @@ class TVBatch(object):
         self.unit_off = torch.zeros(self.B + 1, dtype=torch.int32, device="cuda")
         self.stream_rows = 128
         self.total_units, self.sgrid = 0, 0
+        # "auto" streams while the running instances give at least this many
+        # units (about four wavefronts per SIMD), else takes the tile kernel
+        self.stream_min_units = STREAM_MIN_UNITS
+        self._stream_now = True
         self.set_multi(multi)
         self.set_stream_rows(stream_rows)
         self.set_tile(*tile)
@@ -239,6 +246,7 @@ class TVBatch(object):
         return -(-blocks // 8) * 8
 
     def _use_stream(self, cfg):
+        """Whether the next K-iteration launches stream (else: LDS tiles)."""
         if self.multi_kernel == "tile" or self.multi < 2:
             return False
         if cfg.steps == STEP_MODES["precond"]:
@@ -246,7 +254,12 @@ class TVBatch(object):
                 raise ValueError("the streaming kernel has no preconditioned steps; use multi_kernel='tile' "
                                  "(at most 4 iterations per launch)")
             return False
-        return True
+        return self.multi_kernel == "stream" or self._stream_now
+
+    def _launch_k(self, cfg):
+        """Iterations per K-iteration launch: multi, or at most 4 on the tile
+        kernel (the same iterates, grouped differently)."""
+        return self.multi if self._use_stream(cfg) else min(self.multi, 4)
 
     def set_tile(self, tw, rpt):
         if self.passes == 1:
@@ -307,9 +320,11 @@ class TVBatch(object):
         nrun = int(self.nact.item())
         tiles = int(self.act_off[nrun].item()) if nrun > 0 else 0
         self.grid = self._launch_grid(tiles)
-        if self.multi > 1 and self._grid_override is None:
+        if self.multi > 1:
             units = int(self.unit_off[nrun].item()) if nrun > 0 else 0
-            self.sgrid = self._stream_grid(units)
+            if self._grid_override is None:
+                self.sgrid = self._stream_grid(units)
+            self._stream_now = units >= self.stream_min_units
         return nrun
 
     def _cfg(self, steps):
@@ -423,7 +438,7 @@ class TVBatch(object):
         """k iterations in one launch from buffer p, then the k step updates."""
         P = _hip.ptr
         s = _hip.stream()
-        if self._use_stream(cfg) and k == self.multi:
+        if self._use_stream(cfg) and k == self.multi and k > 1:
             self._call("opy_tv_stream" + self._sfx(), P(self.x[p]), P(self.x[1 - p]), P(self.y[p]),
                        P(self.y[1 - p]), P(self.f), P(self.insts), P(self.state), P(self.active_inst),
                        P(self.unit_off), P(self.nact), ctypes.byref(cfg), int(k), self.stream_rows, self.sgrid, s)
@@ -436,7 +451,7 @@ class TVBatch(object):
 
     def _iterations(self, n, cfg, adapt, params):
         """n iterations starting from buffer self.cur (updates self.cur)."""
-        k = self.multi if not adapt else 1
+        k = self._launch_k(cfg) if not adapt else 1
         while k > 1 and n >= k:
             self._multi_launch(self.cur, cfg, k)
             self.cur ^= 1
@@ -460,7 +475,7 @@ class TVBatch(object):
         block of a solve runs eagerly, which loads its kernels before any
         capture; a new key is captured (that records without running) and
         then replayed."""
-        key = (self.grid, self.sgrid, self.cur)
+        key = (self.grid, self.sgrid, self._stream_now, self.cur)
         if key in cache:
             g, end = cache[key]
             g.replay()

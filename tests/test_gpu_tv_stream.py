@@ -123,6 +123,29 @@ def test_stream_matches_oracle_and_certifies(gpu):
     assert [r["gap"] for r in res["stream"]] == [r["gap"] for r in res["tile"]]
 
 
+def test_auto_switches_to_tiles_when_few_units_run(gpu):
+    """multi_kernel="auto" streams while enough units run and takes the tile
+    kernel once instances drop out (here: as soon as one has certified); the
+    whole solve certifies with the iterations, gaps and iterates of a solve on
+    the tile kernel alone."""
+    out = {}
+    for kernel in ("auto", "tile"):
+        bat, *_ = _batch("float", 4, kernel, seed=3, data="l2", rows=16)
+        if kernel == "auto":
+            bat.stream_min_units = bat.total_units  # stream only while every instance runs
+            bat.solve(steps="accel", step_bound=0.124, max_iters=40, block_iters=40, tol=1e-30)
+            assert bat._use_stream(bat._cfg("accel"))  # every instance still running
+        bat.set_start(None, None)  # cold start for both
+        res = bat.solve(steps="accel", step_bound=0.124, max_iters=20000, block_iters=40, tol=1e-5)
+        out[kernel] = ([r["iter"] for r in res], [r["gap"] for r in res], bat.solution())
+        if kernel == "auto":
+            assert not bat._use_stream(bat._cfg("accel"))
+    assert all(it < 20000 for it in out["auto"][0])
+    assert out["auto"][0] == out["tile"][0] and out["auto"][1] == out["tile"][1]
+    for a, b in zip(out["auto"][2][0] + out["auto"][2][1], out["tile"][2][0] + out["tile"][2][1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_stream_units_follow_the_running_instances(gpu):
     """With some instances marked certified, the compacted unit prefix covers
     exactly the running instances (host recount), the next launches are
@@ -254,7 +277,8 @@ def test_stream_rejects_bad_arguments(gpu):
         call(bat._cfg("accel"), 4, 0)
     with pytest.raises(_hip.HipError, match="rows"):
         bat._call("opy_tv_stream_units", P(bat.insts), P(bat.active_inst), P(bat.nact), P(bat.unit_off), 4, -1, s)
-    # the auto choice keeps preconditioned steps on the tile kernel
+    # the auto choice keeps preconditioned steps on the tile kernel (whatever the unit count)
     auto, *_ = _batch("float", 4, "auto")
+    auto.stream_min_units = 0
     auto.solve(steps="precond", step_bound=0.124, max_iters=8, block_iters=8, tol=1e-30)
     assert not auto._use_stream(auto._cfg("precond")) and auto._use_stream(auto._cfg("accel"))
