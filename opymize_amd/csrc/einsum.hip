@@ -56,6 +56,11 @@ __global__ void __launch_bounds__(kBlock) matmul_r_kernel(const T* __restrict__ 
 }
 
 // ---- FP32 per-point products on the matrix cores -------------------------
+// Three kernels (the launcher picks by layout and size): this direct one, 16
+// points at a time (point-major on small N or K > J); `matmul_mfma_lm_kernel`
+// (label-major) and `matmul_mfma_pm_kernel` (point-major on large N), which
+// stage 64 points per round through LDS.
+//
 // One wavefront forms the products of 16 points at a time as 16 x 16 tiles of
 // v_mfma_f32_16x16x4f32 (FP32 operands, FP32 accumulation), in K/4 steps.
 // Lane l of the wavefront holds the first operand's element (row l%16,
@@ -138,18 +143,225 @@ __global__ void __launch_bounds__(kBlock) matmul_mfma_kernel(const float* __rest
   }
 }
 
+// ---- label-major products with LDS staging ------------------------------
+// MatrixMult (y[j, i] = sum_k A(j, k) x[k, i]): a wavefront takes 64 points at
+// a time, loads their K rows of x as runs of 64 consecutive floats (one round
+// ahead, while the previous round computes and stores), stages them in LDS,
+// forms four 16 x 16 tiles per 16 outputs with v_mfma_f32_16x16x4f32 (the
+// matrix as the first operand, held in registers for the launch), stages the
+// 64 x J results in LDS over the dead x block and stores J runs of 64 floats.
+// Rows k >= K of the staged x are zero, like their matrix entries.
+constexpr int kMmPts = 64;  // points per wavefront and round
+template <int KS, int JT>
+constexpr int mm_lds_floats() {  // per wavefront: the larger of the staged x (KP rows) and y (JP rows), pitch 65
+  return (4 * KS > 16 * JT ? 4 * KS : 16 * JT) * (kMmPts + 1);
+}
+
+template <int KS, int JT>
+__global__ void __launch_bounds__(kBlock) matmul_mfma_lm_kernel(const float* __restrict__ x, float* __restrict__ y,
+                                                                const float* __restrict__ A, int64_t N, int J, int K,
+                                                                int trans, int add) {
+  constexpr int KP = 4 * KS, JP = 16 * JT;
+  constexpr int XP = kMmPts + 1;  // row pitch of the staged blocks (+1: spreads a column's reads over the banks)
+  __shared__ float sbuf[kBlock / 64][mm_lds_floats<KS, JT>()];
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  float* sw = sbuf[threadIdx.x >> 6];
+  float m[KS][JT];  // A(j, k) at k = 4s + lq, j = 16t + lr: A[j*K + k] (A is J x K) or A[k*J + j] (trans)
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int k = 4 * s + lq, j = 16 * t + lr;
+      m[s][t] = (k < K && j < J) ? (trans ? A[k * J + j] : A[j * K + k]) : 0.0f;
+    }
+  const int64_t nrounds = (N + kMmPts - 1) / kMmPts;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  auto load_round = [&](int64_t rd, float (&xr)[KP]) {
+    const int64_t p0 = rd * kMmPts;
+    const bool in = p0 + lane < N;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) xr[k] = (k < K && in) ? x[(int64_t)k * N + p0 + lane] : 0.0f;
+  };
+  int64_t rd = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  float xr[KP];
+  if (rd < nrounds) load_round(rd, xr);
+  for (; rd < nrounds; rd += nwaves) {
+    const int64_t p0 = rd * kMmPts;
+    // staged x: (k, point pt) at sw[k*XP + pt]
+#pragma unroll
+    for (int k = 0; k < KP; ++k) sw[k * XP + lane] = xr[k];
+    if (rd + nwaves < nrounds) load_round(rd + nwaves, xr);
+    opy_f32x4 acc[4][JT];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      float xv[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) xv[s] = sw[(4 * s + lq) * XP + 16 * tt + lr];
+#pragma unroll
+      for (int t = 0; t < JT; ++t) acc[tt][t] = opy_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int t = 0; t < JT; ++t)
+          acc[tt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(m[s][t], xv[s], acc[tt][t], 0, 0, 0);
+    }
+    // staged y: (output j, point pt) at sw[j*XP + pt]; the staged x is dead
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int t = 0; t < JT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sw[(16 * t + 4 * lq + r) * XP + 16 * tt + lr] = acc[tt][t][r];
+    if (p0 + lane < N) {
+      // in groups of 8 rows: the group's LDS reads (and y's, with add) go out before its stores
+#pragma unroll 1
+      for (int g = 0; g < J; g += 8) {
+        float yr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) yr[u] = g + u < J ? sw[(g + u) * XP + lane] : 0.0f;
+        if (add) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (g + u < J) yr[u] += y[(int64_t)(g + u) * N + p0 + lane];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (g + u < J) y[(int64_t)(g + u) * N + p0 + lane] = yr[u];
+      }
+    }
+  }
+}
+
+// ---- point-major products with LDS staging ------------------------------
+// MatrixMultR (y[i, j] = sum_k x[i, k] A(k, j)): the 64 points of a round are
+// one run of 64*K floats of x and one of 64*J floats of y.  A wavefront loads
+// the run (one round ahead), copies it to LDS in the same order, reads the
+// operands at stride K (k >= K read as 0), stages the results at stride J over
+// the dead x run and stores the y run; no index arithmetic per element.
+template <int KS, int JT>
+constexpr int mm_pm_lds_floats() {  // per wavefront: the larger of the two runs
+  return kMmPts * (4 * KS > 16 * JT ? 4 * KS : 16 * JT);
+}
+
+template <int KS, int JT>
+__global__ void __launch_bounds__(kBlock) matmul_mfma_pm_kernel(const float* __restrict__ x, float* __restrict__ y,
+                                                                const float* __restrict__ A, int64_t N, int J, int K,
+                                                                int trans, int add) {
+  constexpr int KP = 4 * KS, JP = 16 * JT;
+  __shared__ float sbuf[kBlock / 64][mm_pm_lds_floats<KS, JT>()];
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lq = lane >> 4;
+  float* sw = sbuf[threadIdx.x >> 6];
+  float m[KS][JT];  // A(k, j) at k = 4s + lq, j = 16t + lr: A[k*J + j] (A is K x J) or A[j*K + k] (trans)
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+#pragma unroll
+    for (int t = 0; t < JT; ++t) {
+      const int k = 4 * s + lq, j = 16 * t + lr;
+      m[s][t] = (k < K && j < J) ? (trans ? A[j * K + k] : A[k * J + j]) : 0.0f;
+    }
+  const int64_t nrounds = (N + kMmPts - 1) / kMmPts;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  // the round's x run in registers: element it of lane = x[p0*K + lane + 64*it], it < K
+  auto load_round = [&](int64_t rd, float (&xr)[KP]) {
+    const int64_t p0 = rd * kMmPts;
+    const int64_t nv = min<int64_t>(kMmPts, N - p0) * K;
+#pragma unroll
+    for (int it = 0; it < KP; ++it) {
+      const int e = lane + kMmPts * it;
+      xr[it] = (it < K && e < nv) ? x[p0 * K + e] : 0.0f;
+    }
+  };
+  int64_t rd = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  float xr[KP];
+  if (rd < nrounds) load_round(rd, xr);
+  for (; rd < nrounds; rd += nwaves) {
+    const int64_t p0 = rd * kMmPts;
+    const int np = (int)min<int64_t>(kMmPts, N - p0);
+#pragma unroll
+    for (int it = 0; it < KP; ++it)
+      if (it < K) sw[lane + kMmPts * it] = xr[it];
+    if (rd + nwaves < nrounds) load_round(rd + nwaves, xr);
+    opy_f32x4 acc[4][JT];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      float xv[KS];
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + lq;
+        xv[s] = k < K ? sw[(16 * tt + lr) * K + k] : 0.0f;
+      }
+#pragma unroll
+      for (int t = 0; t < JT; ++t) acc[tt][t] = opy_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int t = 0; t < JT; ++t)
+          acc[tt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[s], m[s][t], acc[tt][t], 0, 0, 0);
+    }
+    // the y run of the round: (point pt, output j) at sw[pt*J + j]
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int t = 0; t < JT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = 16 * t + lr;
+          if (j < J) sw[(16 * tt + 4 * lq + r) * J + j] = acc[tt][t][r];
+        }
+    float* yb = y + p0 * J;
+    const int nv = np * J;
+#pragma unroll 1
+    for (int g = 0; g < J; g += 8) {
+      float yr[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = lane + kMmPts * (g + u);
+        yr[u] = (g + u < J && e < nv) ? sw[e] : 0.0f;
+      }
+      if (add) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = lane + kMmPts * (g + u);
+          if (g + u < J && e < nv) yr[u] += yb[e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = lane + kMmPts * (g + u);
+        if (g + u < J && e < nv) yb[e] = yr[u];
+      }
+    }
+  }
+}
+
 // the matrix-core path covers K, J <= 64 in FP32; returns false if the shape is outside it
 template <bool PM>
 bool launch_matmul_mfma(const float* x, float* y, const float* A, int64_t N, int J, int K, int trans, int add,
                         hipStream_t st) {
   if (K > 64 || J > 64) return false;
-  const int64_t ntiles = (N + 15) / 16;
-  const int64_t blocks = std::min<int64_t>((ntiles + 3) / 4, 1 << 16);
+  // label-major: the staged kernel (64 points per wavefront and round).  Point-major: the staged
+  // kernel from 2^20 points with K <= J (measured 45-64% of HBM peak there against 43-51% for
+  // the direct one); below that, or with K > J, the direct kernel (16 points at a time, whose
+  // finer grain keeps more wavefronts busy on small N; 45% at 512^2 against 36%)
+  const bool pm_staged = PM && N >= (int64_t(1) << 20) && K <= J;
+  const int64_t units = (PM && !pm_staged) ? (N + 15) / 16 : (N + kMmPts - 1) / kMmPts;
+  const int64_t blocks = std::min<int64_t>((units + 3) / 4, 1 << 16);
   const dim3 g((unsigned)std::max<int64_t>(blocks, 1)), b(kBlock);
   const int ks = K <= 4 ? 1 : K <= 8 ? 2 : K <= 16 ? 4 : K <= 32 ? 8 : 16;
   const int jt = J <= 16 ? 1 : J <= 32 ? 2 : 4;
-#define OPY_MM(KS_, JT_) \
-  hipLaunchKernelGGL((matmul_mfma_kernel<KS_, JT_, PM>), g, b, 0, st, x, y, A, N, J, K, trans, add)
+#define OPY_MM(KS_, JT_)                                                                                      \
+  do {                                                                                                      \
+    if constexpr (PM) {                                                                                     \
+      if (pm_staged)                                                                                        \
+        hipLaunchKernelGGL((matmul_mfma_pm_kernel<KS_, JT_>), g, b, 0, st, x, y, A, N, J, K, trans, add);  \
+      else                                                                                                  \
+        hipLaunchKernelGGL((matmul_mfma_kernel<KS_, JT_, true>), g, b, 0, st, x, y, A, N, J, K, trans, add); \
+    }                                                                                                       \
+    else                                                                                                    \
+      hipLaunchKernelGGL((matmul_mfma_lm_kernel<KS_, JT_>), g, b, 0, st, x, y, A, N, J, K, trans, add);    \
+  } while (0)
 #define OPY_MM_J(KS_)                 \
   switch (jt) {                       \
     case 1: OPY_MM(KS_, 1); break;    \

@@ -106,6 +106,34 @@ def test_matrix_mult(gpu, dtype, shape, N, trans):
         close(host(r), ops.rowwise_lp(M, 1), dtype, scale=10)
 
 
+@pytest.mark.parametrize("shape,trans", [((15, 30), False), ((16, 16), True), ((30, 15), False), ((7, 50), True)])
+def test_matrix_mult_large(gpu, shape, trans):
+    """FP32 on the matrix cores at 2^20 + 37 points (the staged kernels of
+    both layouts, with a partial last round; K > J keeps the direct one)
+    against numpy's einsum in FP64, plain and with add."""
+    import torch
+    from opymize_amd.linear import MatrixMult, MatrixMultR
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal(shape)
+    N = (1 << 20) + 37
+    for cls in (MatrixMult, MatrixMultR):
+        op = cls(N, A, trans=trans)
+        x = rng.standard_normal(op.x.size).astype(np.float32)
+        y0 = rng.standard_normal(op.y.size).astype(np.float32)
+        if cls is MatrixMult:  # y[j, i] = sum_k A(j, k) x[k, i]
+            Ajk = A.T if trans else A
+            ref = np.einsum("jk,ki->ji", Ajk, x.reshape(-1, N).astype(np.float64)).ravel()
+        else:  # y[i, j] = sum_k x[i, k] A(k, j)
+            Akj = A.T if trans else A
+            ref = np.einsum("ik,kj->ij", x.reshape(N, -1).astype(np.float64), Akj).ravel()
+        for add in (False, True):
+            yd = torch.as_tensor(y0.copy()).cuda()
+            op(torch.as_tensor(x).cuda(), yd, add=add)
+            want = ref + (y0.astype(np.float64) if add else 0.0)
+            err = np.abs(yd.cpu().numpy().astype(np.float64) - want).max()
+            assert err < 2e-5 * max(1.0, np.abs(want).max()), (cls.__name__, add, err)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_csr_and_blockop(gpu, dtype):
     from opymize_amd.linear import BlockOp, CsrOp, GradientOp, ScaleOp, IdentityOp, DiagMatrixMultR
