@@ -296,6 +296,28 @@ int opy_fn_eval_f32(int kind, const float* x, int64_t npts, int width, const opy
 int opy_fn_eval_f64(int kind, const double* x, int64_t npts, int width, const opy_fn_params_t* prm, double* work,
                     double* out3, int accumulate, void* stream);
 
+/* ---- semismooth Newton continuation (reference opymize/solvers/semismooth_newton.py) ---- */
+/* out[0] = sum_i x_i y_i in FP64 (the np.einsum('i,i->') products of semismooth_newton.py:107-114,
+ * :291, :314, :322; work: OPY_RED_BLOCKS*3 doubles) */
+int opy_dot_f32(const float* x, const float* y, int64_t n, double* work, double* out, void* stream);
+int opy_dot_f64(const double* x, const double* y, int64_t n, double* work, double* out, void* stream);
+/* z = a*x + b*y, z may be x or y (the vector updates of CG_solve, semismooth_newton.py:102-112, of the
+ * Newton system, :211-219, :240-248, of the residual, :276-284, and of armijo, :39) */
+int opy_axpby_f32(const float* x, const float* y, float* z, int64_t n, double a, double b, void* stream);
+int opy_axpby_f64(const double* x, const double* y, double* z, int64_t n, double a, double b, void* stream);
+/* Jacobian of PosProj (positive=1) / NegProj at xbar applied to x: y (+)= (xbar > 0 / < 0) ? x : 0;
+ * proj.py:25-50, :58-85 (NihilOp with keep from the prox input) */
+int opy_signproj_jac_f32(const float* xbar, const float* x, float* y, int64_t n, int positive, int add,
+                         void* stream);
+int opy_signproj_jac_f64(const double* xbar, const double* x, double* y, int64_t n, int positive, int add,
+                         void* stream);
+/* Jacobian of the Frobenius L1NormsProj at xbar (N points of M entries) applied to x: y (+)= x inside the
+ * ball, (lbd/|xbar|)(x - <u,x> u) with u = xbar/|xbar| outside; L12ProjJacobian, proj.py:228-275 */
+int opy_l12proj_jac_f32(const float* xbar, const float* x, float* y, int64_t N, int M, double lbd, int add,
+                        void* stream);
+int opy_l12proj_jac_f64(const double* xbar, const double* x, double* y, int64_t N, int M, double lbd, int add,
+                        void* stream);
+
 /* ---- fused, batched TV iterations ---------------------------------------
  * PDHG for G + tvw*|grad x|_{2,1} with the forward-difference gradient on 2-D
  * images of C <= 4 channels, many instances of uneven size per launch.  These

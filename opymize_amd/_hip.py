@@ -120,6 +120,10 @@ _SIGS = {
     "clip": [_P, _P, _L, _D, _D, _P],
     "simplex_proj": [_P, _L, _I, _P],
     "block_min": [_P, _L, _I, _P],
+    "dot": [_P, _P, _L, _P, _P, _P],
+    "axpby": [_P, _P, _P, _L, _D, _D, _P],
+    "signproj_jac": [_P, _P, _P, _L, _I, _I, _P],
+    "l12proj_jac": [_P, _P, _P, _L, _I, _D, _I, _P],
 }
 # untyped entry points (no _f32/_f64 suffix)
 _SIGS_PLAIN = {

@@ -85,6 +85,24 @@ int run_reduction(const Op& op, int64_t n, double* work, double* out, int nout, 
   return check_launch(what);
 }
 
+// ---- dot product in FP64 (the conjugate gradients, Armijo slopes and
+// residual norms of semismooth_newton.py:107-114, :291, :314, :322) -------
+template <typename T>
+struct DotOp {
+  const T* x;
+  const T* y;
+  __device__ inline void operator()(int64_t i, double& s, double& m1, double& m2) const {
+    s += (double)x[i] * (double)y[i];
+  }
+};
+
+template <typename T>
+int dot_impl(const T* x, const T* y, int64_t n, double* work, double* out, void* s) {
+  if (n < 0) return fail(kInvalid, "dot: negative size");
+  if (n > 0 && (!x || !y)) return fail(kInvalid, "dot: null operand");
+  return run_reduction(DotOp<T>{x, y}, n, work, out, 1, 0, s, "dot");
+}
+
 // ---- PDHG residuals (reference pdhg.py:74-78, :111-116, :155-164) -------
 // sums of ((z - zn)/a - b*g + c1*gn)^2 and ((z - zn)/a - b*g + c2*gn)^2
 template <typename T>
@@ -554,6 +572,13 @@ OPY_API int opy_pdhg_residual2_f64(const opy_step_t* a, const double* z, const d
                                    const double* g, double c1, double c2, const double* gn, int64_t n,
                                    double* work, double* out2, void* s) {
   return residual2_impl<double>(a, z, zn, b, g, c1, c2, gn, n, work, out2, s);
+}
+
+OPY_API int opy_dot_f32(const float* x, const float* y, int64_t n, double* work, double* out, void* s) {
+  return dot_impl<float>(x, y, n, work, out, s);
+}
+OPY_API int opy_dot_f64(const double* x, const double* y, int64_t n, double* work, double* out, void* s) {
+  return dot_impl<double>(x, y, n, work, out, s);
 }
 
 OPY_API int opy_pdhg_adapt(opy_adapt_state_t* st, const opy_adapt_params_t* p, void* s) {

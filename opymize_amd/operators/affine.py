@@ -149,9 +149,28 @@ class SSDProxOp(Operator):
         self.y = ssd.x
         self.tau = StepArg(tau)
 
+    def jacobian(self):
+        """diag(a), a = 1/(1 + tau vol) on the data points and 1 elsewhere: the
+        Jacobian of the reference's ShiftScaleOp form of this prox
+        (reference: quad.py:45-55, affine.py:94, :133-135)."""
+        s = self.ssd
+        N, C = s.f.shape
+        t = self.tau.tau
+        if isinstance(t, tuple):  # (device FP64 tensor, index)
+            t = float(t[0][t[1]].item())
+        elif isinstance(t, torch.Tensor):
+            t = t.detach().cpu().numpy()
+        t = np.asarray(t, dtype=np.float64)
+        tv = (np.full((N, C), float(t.ravel()[0])) if t.size == 1 else t.reshape(N, C)) * s.vol[None, :]
+        tv[~s.mask, :] = 0.0
+        key = tv.tobytes()
+        if getattr(self, "_jac_key", None) != key:
+            self._jac = ScaleOp(N * C, (1.0 / (1.0 + tv)).ravel())
+            self._jac_key = key
+        return self._jac
+
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
-        if jacobian:
-            raise NotImplementedError("jacobians of prox maps are not part of the PDHG hot path")
+        jac = self.jacobian() if jacobian else None
         y = _out(x, y)
         s = self.ssd
         N, C = s.f.shape
@@ -159,6 +178,7 @@ class SSDProxOp(Operator):
         st = self.tau.step(x.dtype)
         _hip.call("ssd_prox", x.dtype, _hip.ptr(x), _hip.ptr(y), N, C, _hip.ptr(f), _hip.ptr(vol),
                   _hip.ptr(mask), _hip.ctypes.byref(st), int(add), _hip.stream())
+        return jac
 
 
 class L1DataProxOp(Operator):
@@ -176,7 +196,8 @@ class L1DataProxOp(Operator):
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
         if jacobian:
-            raise NotImplementedError("jacobians of prox maps are not part of the PDHG hot path")
+            raise NotImplementedError("L1DataProxOp: no Jacobian (the TV-L1 data term has no reference "
+                                      "counterpart, and its prox is not differentiable at f +- t lbd)")
         y = _out(x, y)
         N, C = self.fct.f.shape
         f, lbd = self.fct._dev(x.dtype)

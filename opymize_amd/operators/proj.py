@@ -2,7 +2,7 @@
 import numpy as np
 import torch
 
-from opymize_amd import Operator, Variable, _hip
+from opymize_amd import LinOp, Operator, Variable, _hip
 from opymize_amd._devcache import DeviceConstants
 
 
@@ -14,6 +14,52 @@ def _inplace_target(x, y):
     return y
 
 
+class _PointJacobian(LinOp):
+    """Jacobian of a pointwise projection at the point it was last applied
+    to: keeps a device copy of that input (the projections run in place)."""
+
+    def __init__(self, N):
+        LinOp.__init__(self)
+        self.x = Variable(N)
+        self.y = Variable(N)
+        self.adjoint = self  # projections have symmetric Jacobians
+        self.xbar = None
+
+    def update(self, xbar):
+        if self.xbar is None or self.xbar.dtype != xbar.dtype or self.xbar.numel() != xbar.numel() \
+                or self.xbar.device != xbar.device:
+            self.xbar = torch.empty(xbar.numel(), dtype=xbar.dtype, device=xbar.device)
+        _hip.call("copy", xbar.dtype, _hip.ptr(xbar), _hip.ptr(self.xbar), xbar.numel(), _hip.stream())
+        return self
+
+    def _point(self, x, y):
+        """The stored point, after checking x and the output y against it."""
+        if self.xbar is None:
+            raise RuntimeError("%s: no point to linearize at (apply the projection with jacobian=True)"
+                               % type(self).__name__)
+        for t in (x, y):
+            if t.numel() != self.xbar.numel() or t.dtype != self.xbar.dtype:
+                raise ValueError("%s: %d entries of %s for a Jacobian at %d entries of %s"
+                                 % (type(self).__name__, t.numel(), t.dtype, self.xbar.numel(), self.xbar.dtype))
+        return self.xbar
+
+
+class SignProjJacobian(_PointJacobian):
+    """Jacobian of PosProj / NegProj at xbar: keeps x where xbar > 0 (< 0),
+    zero elsewhere (reference: proj.py:25, :38-40: NihilOp with keep taken
+    from the input)."""
+
+    def __init__(self, N, positive):
+        _PointJacobian.__init__(self, N)
+        self.positive = bool(positive)
+
+    def _call_gpu(self, x, y=None, add=False):
+        y = x if y is None else y
+        xb = self._point(x, y)
+        _hip.call("signproj_jac", x.dtype, _hip.ptr(xb), _hip.ptr(x), _hip.ptr(y), x.numel(), int(self.positive),
+                  int(add), _hip.stream())
+
+
 class PosProj(Operator):
     """T(x) = max(0, x) (reference: proj.py:19-52)."""
 
@@ -21,12 +67,13 @@ class PosProj(Operator):
         Operator.__init__(self)
         self.x = Variable(N)
         self.y = Variable(N)
+        self._jacobian = SignProjJacobian(N, True)
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
-        if jacobian:
-            raise NotImplementedError("jacobians of projections are not part of the PDHG hot path")
+        jac = self._jacobian.update(x) if jacobian else None
         y = x if y is None else y
         _hip.call("signproj", x.dtype, _hip.ptr(x), _hip.ptr(y), x.numel(), 1, int(add), _hip.stream())
+        return jac
 
 
 class NegProj(Operator):
@@ -36,12 +83,31 @@ class NegProj(Operator):
         Operator.__init__(self)
         self.x = Variable(N)
         self.y = Variable(N)
+        self._jacobian = SignProjJacobian(N, False)
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
-        if jacobian:
-            raise NotImplementedError("jacobians of projections are not part of the PDHG hot path")
+        jac = self._jacobian.update(x) if jacobian else None
         y = x if y is None else y
         _hip.call("signproj", x.dtype, _hip.ptr(x), _hip.ptr(y), x.numel(), 0, int(add), _hip.stream())
+        return jac
+
+
+class L12ProjJacobian(_PointJacobian):
+    """Jacobian of the Frobenius L1NormsProj at xbar (reference:
+    proj.py:228-275): identity on points inside the ball, and
+    (lbd/|xbar_i|)(I - u u^T), u = xbar_i/|xbar_i|, outside."""
+
+    def __init__(self, N, M, lbd):
+        _PointJacobian.__init__(self, N * M[0] * M[1])
+        self.x = Variable((N, M[0] * M[1]))
+        self.y = self.x
+        self.N, self.M, self.lbd = int(N), tuple(M), float(lbd)
+
+    def _call_gpu(self, x, y=None, add=False):
+        y = x if y is None else y
+        xb = self._point(x, y)
+        _hip.call("l12proj_jac", x.dtype, _hip.ptr(xb), _hip.ptr(x), _hip.ptr(y), self.N, self.M[0] * self.M[1],
+                  self.lbd, int(add), _hip.stream())
 
 
 class L1NormsProj(Operator):
@@ -66,13 +132,22 @@ class L1NormsProj(Operator):
         self.y = self.x
         self.lbd = float(lbd)
         self.matrixnorm = matrixnorm
+        # a ball of vectors is the same for both norms
+        frob = matrixnorm == "frobenius" or min(M) == 1
+        self._jacobian = L12ProjJacobian(self.N, M, self.lbd) if frob else None
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
         assert not add
-        assert not jacobian
+        jac = None
+        if jacobian:
+            # the reference has the Jacobian of the Frobenius ball only (proj.py:211)
+            if self._jacobian is None:
+                raise NotImplementedError("L1NormsProj: no Jacobian for the spectral ball")
+            jac = self._jacobian.update(x)
         t = _inplace_target(x, y)
         _hip.call("l1norms_proj", t.dtype, _hip.ptr(t), self.N, self.M[0], self.M[1], self.lbd,
                   int(self.matrixnorm == "spectral"), _hip.stream())
+        return jac
 
 
 class QuadEpiProj(Operator):

@@ -25,6 +25,11 @@ transposed operand fails a check that does not just retype the formula):
   lsq_linear on the dual), a hand-derived two-iteration known answer, and
   the strong-convexity bound 0.5*lambda*|x - x*|^2 <= gap.
 
+* newton (semismooth Newton continuation): the prox Jacobians and the
+  Newton matrix against central differences of the prox maps and of the
+  fixed-point residual, CG against a direct solve, Armijo against a
+  hand-worked backtracking, and descent toward the exact 1-D TV solution.
+
 Parity unpinned: the exact adaptive step trajectory at rounding level
 (reference pdhg.py:171,178 branch on FP comparisons; summation order of the
 reference's PyCUDA reductions is not fixed by any reference test) -- compared
