@@ -42,6 +42,17 @@ __global__ void copy_kernel(const T* __restrict__ x, T* __restrict__ y, int64_t 
   if (i < n) y[i] = x[i];
 }
 
+// Copy with 16-byte accesses when both arrays are 16-byte aligned: thread i
+// moves chunk i of the nv whole chunks, and the first n - head threads also
+// one element of the tail [head, n) (fewer than 16 bytes).
+template <typename T>
+__global__ void copy16_kernel(const uint4* __restrict__ x, uint4* __restrict__ y, int64_t nv,
+                              const T* __restrict__ xt, T* __restrict__ yt, int64_t head, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) y[i] = x[i];
+  if (i < n - head) yt[head + i] = xt[head + i];
+}
+
 // y (+)= fact * x (reference: ScaleOp, scale.py:48-62; IdentityOp with fact 1)
 template <typename T>
 __global__ void scale_kernel(const T* __restrict__ x, T* __restrict__ y, int64_t n, const T* __restrict__ fa,
@@ -244,7 +255,16 @@ OPY_API int opy_fill_f64(double* y, int64_t n, double v, void* s) { return fill_
 template <typename T>
 static int copy_impl(const T* x, T* y, int64_t n, void* s) {
   if (int rc = check_n<T>(x, y, n, "copy")) return rc;
-  if (x != y) OPY_LAUNCH((copy_kernel<T>), n, s, x, y, n);
+  if (x == y || n == 0) return kOk;
+  if (reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0) {
+    constexpr int64_t per = 16 / sizeof(T);
+    const int64_t nv = n / per;
+    const int64_t head = nv * per;
+    OPY_LAUNCH((copy16_kernel<T>), (nv > 0 ? nv : 1), s, reinterpret_cast<const uint4*>(x),
+               reinterpret_cast<uint4*>(y), nv, x, y, head, n);
+  } else {
+    OPY_LAUNCH((copy_kernel<T>), n, s, x, y, n);
+  }
   return check_launch("copy");
 }
 OPY_API int opy_copy_f32(const float* x, float* y, int64_t n, void* s) { return copy_impl<float>(x, y, n, s); }

@@ -2,7 +2,7 @@
 the matrix cores, for profiles: each shape and size runs a few warm-up
 applies and then `reps` timed ones (HIP events on the launch stream), and
 prints the time and the algorithmic bandwidth, N * (K + J) * 4 bytes per
-apply.
+apply; after each shape, the library's copy kernel moving the same bytes.
 
 Usage: python -m opymize_amd.tools.einsum_bench [--sizes N ...] [--reps R]
 """
@@ -16,6 +16,7 @@ HBM_PEAK_GBS = 8000.0
 
 def main(argv=None):
     import torch
+    from opymize_amd import _hip
     from opymize_amd.linear import MatrixMult, MatrixMultR
     ap = argparse.ArgumentParser()
     ap.add_argument("--sizes", type=int, nargs="+", default=[512 * 512, 4096 * 1024])
@@ -46,6 +47,24 @@ def main(argv=None):
                 print("%-11s N=%9d K=%2d J=%2d: %8.1f us  %6.0f GB/s  %.2f of peak"
                       % (cls.__name__, N, K, J, 1e3 * ms, gbs, gbs / HBM_PEAK_GBS), flush=True)
                 del x, y, op
+            # the library's copy kernel moving the same bytes (N (K + J) / 2 floats read and
+            # written): what a launch of this size reaches without arithmetic
+            n = N * (K + J) // 2
+            a_, b_ = torch.ones(n, device="cuda"), torch.empty(n, device="cuda")
+            cp = lambda: _hip.call("copy", a_.dtype, _hip.ptr(a_), _hip.ptr(b_), n, _hip.stream())
+            for _ in range(3):
+                cp()
+            torch.cuda.synchronize()
+            e0.record(st)
+            for _ in range(a.reps):
+                cp()
+            e1.record(st)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / a.reps
+            gbs = 2 * n * 4 / (ms * 1e-3) / 1e9
+            print("%-11s N=%9d K=%2d J=%2d: %8.1f us  %6.0f GB/s  %.2f of peak"
+                  % ("copy", N, K, J, 1e3 * ms, gbs, gbs / HBM_PEAK_GBS), flush=True)
+            del a_, b_
 
 
 if __name__ == "__main__":

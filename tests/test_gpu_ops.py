@@ -304,6 +304,24 @@ def test_normest_matches_oracle(gpu):
     assert 7.0 < e_gpu ** 2 <= 8.0
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_copy_sizes_and_alignments(gpu, dtype):
+    """opy_copy: 16-byte chunks plus a tail when both arrays are 16-byte
+    aligned, one element per thread otherwise; every size and offset gives
+    the source bit for bit and writes nothing past the end."""
+    import torch
+    from opymize_amd import _hip
+    rng = np.random.default_rng(12)
+    src = torch.as_tensor(rng.standard_normal(2 ** 20 + 64).astype(dtype)).cuda()
+    for n in (0, 1, 3, 4, 5, 255, 1023, 1025, 2 ** 20 + 3):
+        for xo, yo in ((0, 0), (1, 1), (0, 1), (2, 0)):
+            dst = torch.full((n + 16,), -7.0, dtype=src.dtype, device="cuda")
+            _hip.call("copy", src.dtype, _hip.addr(src, xo), _hip.addr(dst, yo), n, _hip.stream())
+            d = dst.cpu().numpy()
+            assert np.array_equal(d[yo:yo + n], src[xo:xo + n].cpu().numpy()), (n, xo, yo)
+            assert (d[:yo] == -7).all() and (d[yo + n:] == -7).all(), (n, xo, yo)
+
+
 def test_staged_host_arrays(gpu):
     """numpy inputs are staged through the GPU kernels (same results)."""
     from opymize_amd.linear import GradientOp
