@@ -29,6 +29,7 @@ NegProj, ConstOp, ConstrainOp, ShiftScaleOp, L1NormsProj (Frobenius), SSD's
 prox, SplitOp of those, and every LinOp (itself).
 """
 import logging
+import math
 
 import numpy as np
 import torch
@@ -386,5 +387,203 @@ class SemismoothNewton(object):
         return tuple(t.cpu().numpy() for t in self._split(self.itervars["xyk"]))
 
 
+def _norm(v):
+    """Euclidean norm as numpy's norm forms it for a vector: sqrt(<v, v>)."""
+    d = _dot(v, v)
+    return math.sqrt(d) if d >= 0 else float("nan")
+
+
+def _scalar_search_armijo(phi, phi0, derphi0, c1=1e-4, alpha0=1.0, amin=0.0):
+    """Backtracking with quadratic, then cubic interpolation (Nocedal & Wright,
+    Numerical Optimization, 1999, pp. 56-57), as scipy's line search of the
+    same name runs it (scipy 1.15 optimize/_linesearch.py:684-739).  Returns
+    (alpha or None, phi there)."""
+    phi_a0 = phi(alpha0)
+    if phi_a0 <= phi0 + c1 * alpha0 * derphi0:
+        return alpha0, phi_a0
+    # minimizer of the quadratic through phi0, derphi0 and phi_a0
+    alpha1 = -(derphi0) * alpha0 ** 2 / 2.0 / (phi_a0 - phi0 - derphi0 * alpha0)
+    phi_a1 = phi(alpha1)
+    if phi_a1 <= phi0 + c1 * alpha1 * derphi0:
+        return alpha1, phi_a1
+    # then the minimizers of cubics through phi0, derphi0 and the last two values
+    while alpha1 > amin:
+        factor = alpha0 ** 2 * alpha1 ** 2 * (alpha1 - alpha0)
+        a = alpha0 ** 2 * (phi_a1 - phi0 - derphi0 * alpha1) - alpha1 ** 2 * (phi_a0 - phi0 - derphi0 * alpha0)
+        a = a / factor
+        b = -alpha0 ** 3 * (phi_a1 - phi0 - derphi0 * alpha1) + alpha1 ** 3 * (phi_a0 - phi0 - derphi0 * alpha0)
+        b = b / factor
+        alpha2 = (-b + math.sqrt(abs(b ** 2 - 3 * a * derphi0))) / (3.0 * a)
+        phi_a2 = phi(alpha2)
+        if phi_a2 <= phi0 + c1 * alpha2 * derphi0:
+            return alpha2, phi_a2
+        if (alpha1 - alpha2) > alpha1 / 2.0 or (1 - alpha2 / alpha1) < 0.96:
+            alpha2 = alpha1 / 2.0
+        alpha0, alpha1 = alpha1, alpha2
+        phi_a0, phi_a1 = phi_a1, phi_a2
+    return None, phi_a1
+
+
+class _LowRankInverse(object):
+    """G = g I + sum_n c_n d_n^T on device vectors (scipy 1.15
+    optimize/_nonlin.py:614-700, LowRankMatrix without the dense collapse,
+    which only a rank above the vector length would trigger)."""
+
+    def __init__(self, g):
+        self.g = g
+        self.cs, self.ds = [], []
+
+    def matvec(self, v, out):
+        """out = G v"""
+        _axpby(v, v, out, self.g, 0.0)
+        for c, d in zip(self.cs, self.ds):
+            _axpby(c, out, out, _dot(d, v), 1.0)
+
+    def rmatvec(self, v, out):
+        """out = G^T v"""
+        _axpby(v, v, out, self.g, 0.0)
+        for c, d in zip(self.cs, self.ds):
+            _axpby(d, out, out, _dot(c, v), 1.0)
+
+
+def broyden1(F, x0, iters):
+    """`iters` steps of Broyden's first ("good") method with Armijo line
+    search from x0, as scipy.optimize.broyden1(F, x0, iter=iters) takes them
+    with its defaults (scipy 1.15 optimize/_nonlin.py: nonlin_solve :122-270,
+    _nonlin_line_search :272-315, TerminationCondition :317-375, GenericBroyden
+    :588-612, BroydenFirst :842-905): the inverse Jacobian approximation
+    starts at -alpha I with alpha = 0.5 max(|x0|, 1)/|F(x0)| and takes one
+    rank-one term per step.  F(x, out) writes F(x) into out.  Returns x on
+    the device.
+
+    A non-finite step restarts the approximation at -alpha I, as scipy's does;
+    here the test is on <r, r> (one FP64 reduction), which differs from
+    scipy's entrywise test only for entries beyond 1e154, and d = v/<df, v>
+    is formed as a product with the reciprocal (one rounding more)."""
+    x = x0.clone()
+    Fx = torch.empty_like(x)
+    F(x, Fx)
+
+    last_x, last_f = x.clone(), Fx.clone()
+    nf = _norm(Fx)
+    alpha = 0.5 * max(_norm(x), 1) / nf if nf else 1.0  # set once; a restart keeps it
+    G = _LowRankInverse(-alpha)
+    dx = torch.empty_like(x)
+    it = 0
+    while True:
+        it += 1  # TerminationCondition.check with iter given: stop on F = 0 or after `iters` steps
+        if _dot(Fx, Fx) == 0 or it > iters:
+            break
+        G.matvec(Fx, dx)
+        if not math.isfinite(_dot(dx, dx)):
+            G = _LowRankInverse(-alpha)
+            G.matvec(Fx, dx)
+        _axpby(dx, dx, dx, -1.0, 0.0)  # dx = -G F(x)
+        if _norm(dx) == 0:
+            raise ValueError("Jacobian inversion yielded zero vector. This indicates a bug in the Jacobian "
+                             "approximation.")
+        # line search on phi(s) = |F(x + s dx)|^2 (scipy _nonlin_line_search, 'armijo')
+        store = {"s": 0.0, "phi": _norm(Fx) ** 2, "F": Fx}
+
+        def phi(sv):
+            if sv == store["s"]:
+                return store["phi"]
+            xt = torch.empty_like(x)
+            _axpby(x, dx, xt, 1.0, sv)
+            v = torch.empty_like(x)
+            F(xt, v)
+            d = _dot(v, v)
+            p = math.sqrt(d) ** 2 if math.isfinite(d) else math.inf
+            store.update(s=sv, phi=p, F=v)
+            return p
+
+        sv, _ = _scalar_search_armijo(phi, store["phi"], -store["phi"], amin=1e-2)
+        if sv is None:
+            sv = 1.0
+        _axpby(x, dx, x, 1.0, sv)
+        if sv == store["s"]:
+            Fx = store["F"]
+        else:
+            Fx = torch.empty_like(x)
+            F(x, Fx)
+        # Broyden update of the inverse: H+ = H + (dx - H df) dx^T H / (dx^T H df)
+        df = torch.empty_like(x)
+        _axpby(Fx, last_f, df, 1.0, -1.0)
+        ddx = torch.empty_like(x)
+        _axpby(x, last_x, ddx, 1.0, -1.0)
+        v = torch.empty_like(x)
+        G.rmatvec(ddx, v)
+        c = torch.empty_like(x)
+        G.matvec(df, c)
+        _axpby(ddx, c, c, 1.0, -1.0)
+        d = v
+        _axpby(v, v, d, 1.0 / _dot(df, v), 0.0)
+        G.cs.append(c)
+        G.ds.append(d)
+        last_f = Fx
+        last_x = x.clone()
+    return x
+
+
+class SemismoothQuasinewton(SemismoothNewton):
+    """The fixed-point residual of PDHG solved by Broyden's first method in
+    rounds of `granularity` steps, each round from a fresh Jacobian
+    approximation and followed by a gap check (reference:
+    semismooth_newton.py:429-500, where each round is one call of
+    scipy.optimize.broyden1)."""
+
+    def solve(self, continue_at=None, granularity=10, term_relgap=1e-5, term_infeas=None, term_maxiter=int(1e2),
+              precision="double"):
+        _hip.require_device()
+        i = self.itervars
+        dtype = _hip.torch_dtype(precision)
+        if continue_at is not None:
+            i["xyk"] = _flat_start(continue_at, self.xy.size, dtype)
+        elif i.get("xyk") is None or not _hip.is_device_tensor(i["xyk"]):
+            i["xyk"] = torch.zeros(self.xy.size, dtype=dtype, device="cuda")
+        else:
+            i["xyk"] = i["xyk"].to(dtype)
+        i["xykp1"] = i["xyk"].clone()
+        i["xykres"] = i["xyk"].clone()
+        xk, yk = self._split(i["xyk"])
+        xkp1, ykp1 = self._split(i["xykp1"])
+
+        self.prepare_stepsizes()
+
+        def f(x, out):
+            self.res(x, out, norm=False)
+
+        if term_infeas is None:
+            term_infeas = term_relgap
+        obj_p = obj_d = infeas_p = infeas_d = relgap = 0.
+        self.checks = []
+        logging.info("Solving (steps<%d)..." % term_maxiter)
+        with GracefulInterruptHandler() as interrupt_hdl:
+            _iter = 0
+            while _iter < term_maxiter:
+                _copy(i["xyk"], i["xykp1"])
+                _copy(broyden1(f, i["xykp1"], granularity), i["xyk"])
+                _iter += granularity
+                if interrupt_hdl.interrupted or _iter % granularity == 0:
+                    if interrupt_hdl.interrupted:
+                        print("Interrupt (SIGINT) at iter=%d" % _iter)
+                    self.linop(xk, ykp1)
+                    self.linop.adjoint(yk, xkp1)
+                    obj_p, infeas_p = self.obj_primal(xk, ykp1)
+                    obj_d, infeas_d = self.obj_dual(xkp1, yk)
+                    relgap = (obj_p - obj_d) / max(np.spacing(1), obj_d)
+                    self.checks.append({"iter": _iter, "objp": obj_p, "objd": obj_d, "infeasp": infeas_p,
+                                        "infeasd": infeas_d, "relgap": relgap})
+                    logging.info("#{:6d}: objp = {: 9.6g} ({: 9.6g}), objd = {: 9.6g} ({: 9.6g}), "
+                                 "gap = {: 9.6g}, relgap = {: 9.6g} ".format(
+                                     _iter, obj_p, infeas_p, obj_d, infeas_d, obj_p - obj_d, relgap))
+                    if np.abs(relgap) < term_relgap and max(infeas_p, infeas_d) < term_infeas:
+                        break
+                    if interrupt_hdl.interrupted:
+                        break
+        self.iterations = _iter
+        return {"objp": obj_p, "objd": obj_d, "infeasp": infeas_p, "infeasd": infeas_d, "relgap": relgap}
+
+
 __all__ = ["armijo", "SymRegOp", "CG_solve", "SemismoothNewtonSystem", "SemismoothNewtonSystemAdjoint",
-           "SemismoothNewton"]
+           "SemismoothNewton", "SemismoothQuasinewton", "broyden1"]

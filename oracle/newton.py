@@ -181,3 +181,17 @@ def gap(prob, xy):
     objp, objd = gv + fv, -gcv - fcv
     return {"objp": objp, "objd": objd, "infeasp": gi + fi, "infeasd": gci + fci,
             "relgap": (objp - objd) / max(np.spacing(1), objd)}
+
+
+def quasinewton_solve(prob, xy0, tau, sigma, granularity, rounds):
+    """Rounds of scipy.optimize.broyden1(F, xy, iter=granularity) on the
+    fixed-point residual, a fresh call per round, with the gap after each
+    (reference semismooth_newton.py:429-500, which calls the same scipy
+    routine: the oracle here is that library routine).  Returns (xy, gaps)."""
+    import scipy.optimize
+    xy = np.array(xy0, dtype=np.float64)
+    gaps = []
+    for _ in range(rounds):
+        xy = scipy.optimize.broyden1(lambda v: residual(prob, v, tau, sigma)[0], xy, iter=granularity)
+        gaps.append(gap(prob, xy))
+    return xy, gaps

@@ -264,3 +264,33 @@ def test_newton_refuses_bad_starts_and_models(gpu):
     G1 = L1Data(f, 2.0)
     with pytest.raises(NotImplementedError):
         SemismoothNewton(G1, L1Norms(30, (1, 2), 1.0), GradientOp((6, 5), 1)).solve(term_maxiter=1)
+
+
+def test_quasinewton_rounds_match_scipy(gpu):
+    """SemismoothQuasinewton: rounds of Broyden steps on the device against
+    scipy.optimize.broyden1 on the oracle's residual (what the reference
+    calls), from a rough PDHG state: the iterate after one round of 3 steps
+    and two rounds of 2, and the gap reported after each round.  Broyden's
+    line search branches on comparisons that amplify rounding differences,
+    so the tolerance is 1e-9 relative to the largest entry."""
+    from opymize_amd.solvers import SemismoothQuasinewton
+    W, lam = 40, 4.0
+    rng = np.random.default_rng(4)
+    f = (np.repeat(rng.random(5), 8) + 0.05 * rng.standard_normal(W)).reshape(-1, 1)
+    G, F, A = tv_model(f, (1, W), lam)
+    x0, y0, _, _ = opd.solve(opd.tv_problem(f, (1, W), lam), steps="const", step_bound=0.124, term_maxiter=200,
+                             granularity=1000, term_pd_gap=1e-14)
+    prob = onw.tv_newton_problem(f, (1, W), lam)
+    for gran, rounds in ((3, 1), (2, 2)):
+        s = SemismoothQuasinewton(G, F, A)
+        info = s.solve(continue_at=(x0, y0), granularity=gran, term_maxiter=gran * rounds, term_relgap=0.0)
+        xy, gaps = onw.quasinewton_solve(prob, np.concatenate([x0, y0]), s.constvars["tau"], s.constvars["sigma"],
+                                         gran, rounds)
+        x, y = s.state
+        scale = max(1.0, np.abs(xy).max())
+        assert np.abs(np.concatenate([x, y]) - xy).max() <= 1e-9 * scale, (gran, rounds)
+        assert len(s.checks) == rounds and s.iterations == gran * rounds
+        for c, g in zip(s.checks, gaps):
+            for k in ("objp", "objd", "infeasd"):
+                assert abs(c[k] - g[k]) <= 1e-8 * max(1.0, abs(g[k])), (gran, k)
+        assert info["relgap"] == s.checks[-1]["relgap"]
