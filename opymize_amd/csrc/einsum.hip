@@ -341,13 +341,16 @@ template <bool PM>
 bool launch_matmul_mfma(const float* x, float* y, const float* A, int64_t N, int J, int K, int trans, int add,
                         hipStream_t st) {
   if (K > 64 || J > 64) return false;
-  // label-major: the staged kernel (64 points per wavefront and round).  Point-major: the staged
-  // kernel from 2^20 points with K <= J (measured 45-64% of HBM peak there against 43-51% for
-  // the direct one); below that, or with K > J, the direct kernel (16 points at a time, whose
-  // finer grain keeps more wavefronts busy on small N; 45% at 512^2 against 36%)
-  const bool pm_staged = PM && N >= (int64_t(1) << 20) && K <= J;
+  // label-major: the staged kernel (64 points per wavefront and round), a wavefront per round.
+  // Point-major: from 2^20 points the staged kernel on at most 2048 blocks, so that each
+  // wavefront takes several rounds, with the next round's loads in flight and the matrix loaded
+  // once (at 4M points 0.58-0.64 of HBM peak, against 0.48-0.54 for the direct kernel and for the
+  // staged one with a wavefront per round); below that the direct kernel (16 points at a time,
+  // whose finer grain keeps more wavefronts busy on small N; 0.44-0.47 at 512^2 for 15 x 30 and
+  // 30 x 15, against 0.35-0.36 staged)
+  const bool pm_staged = PM && N >= (int64_t(1) << 20);
   const int64_t units = (PM && !pm_staged) ? (N + 15) / 16 : (N + kMmPts - 1) / kMmPts;
-  const int64_t blocks = std::min<int64_t>((units + 3) / 4, 1 << 16);
+  const int64_t blocks = std::min<int64_t>((units + 3) / 4, pm_staged ? 2048 : (1 << 16));
   const dim3 g((unsigned)std::max<int64_t>(blocks, 1)), b(kBlock);
   const int ks = K <= 4 ? 1 : K <= 8 ? 2 : K <= 16 ? 4 : K <= 32 ? 8 : 16;
   const int jt = J <= 16 ? 1 : J <= 32 ? 2 : 4;
