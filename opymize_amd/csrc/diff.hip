@@ -122,6 +122,105 @@ __global__ void __launch_bounds__(kBlock) div_kernel(const T* __restrict__ p, T*
   *out = add ? *out + val : val;
 }
 
+// The forward gradient and its adjoint on a 2-D grid (H x W, C channels) with
+// one thread per (column, channel) of row blockIdx.y: the arithmetic of
+// grad_kernel<T, false> / div_kernel<T, false> in the same order, without
+// the per-element coordinate divisions of the n-D kernels.  CT > 0 fixes C;
+// for C = 1 the pair (d/drow, d/dcol) of a point is one 2-vector access.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+template <typename T, int CT>
+__global__ void __launch_bounds__(kBlock) grad2f_kernel(const T* __restrict__ x, T* __restrict__ y, int H, int W,
+                                                        int Cr, const double* __restrict__ w, double h0, double h1,
+                                                        int add) {
+  const int C = CT > 0 ? CT : Cr;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= W * C) return;
+  const int r = blockIdx.y;
+  const int c = e / C;
+  const int k = e - c * C;
+  const int i = r * W + c;
+  const double wk = w ? w[k] : 1.0;
+  const T x0 = x[i * C + k];
+  T v0 = T(0), v1 = T(0);
+  if (r < H - 1) v0 = (T)(wk / h0) * (x[(i + W) * C + k] - x0);
+  if (c < W - 1) v1 = (T)(wk / h1) * (x[(i + 1) * C + k] - x0);
+  if constexpr (CT == 1) {
+    using V2 = typename Vec2<T>::type;
+    V2* out = reinterpret_cast<V2*>(y) + i;
+    V2 o;
+    if (add) {
+      o = *out;
+      o.x += v0;
+      o.y += v1;
+    } else {
+      o.x = v0;
+      o.y = v1;
+    }
+    *out = o;
+  } else {
+    T* out = y + (i * 2) * C + k;
+    out[0] = add ? out[0] + v0 : v0;
+    out[C] = add ? out[C] + v1 : v1;
+  }
+}
+
+template <typename T, int CT>
+__global__ void __launch_bounds__(kBlock) div2f_kernel(const T* __restrict__ p, T* __restrict__ x, int H, int W,
+                                                       int Cr, const double* __restrict__ w, double h0, double h1,
+                                                       int add) {
+  const int C = CT > 0 ? CT : Cr;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= W * C) return;
+  const int r = blockIdx.y;
+  const int c = e / C;
+  const int k = e - c * C;
+  const int i = r * W + c;
+  const double wk = w ? w[k] : 1.0;
+  const T ih0 = (T)(1.0 / h0), ih1 = (T)(1.0 / h1);
+  T p0, p1;
+  if constexpr (CT == 1) {
+    using V2 = typename Vec2<T>::type;
+    const V2 pv = reinterpret_cast<const V2*>(p)[i];
+    p0 = pv.x;
+    p1 = pv.y;
+  } else {
+    p0 = p[(i * 2) * C + k];
+    p1 = p[(i * 2 + 1) * C + k];
+  }
+  T acc = T(0);
+  if (r < H - 1) acc -= p0 * ih0;
+  if (r > 0) acc += p[((i - W) * 2) * C + k] * ih0;
+  if (c < W - 1) acc -= p1 * ih1;
+  if (c > 0) acc += p[((i - 1) * 2 + 1) * C + k] * ih1;
+  const T val = (T)wk * acc;
+  T* out = x + i * C + k;
+  *out = add ? *out + val : val;
+}
+
+// The 2-D kernels serve forward 2-D grids whose indices fit 32 bits and whose
+// rows fit the grid's second dimension.
+inline bool use_2d_forward(const Grid& g) {
+  return g.ndims == 2 && g.dims[0] >= 1 && g.dims[0] <= 65535 && g.npoints * g.nchan * 2 < (int64_t(1) << 31);
+}
+
+// vec: the field (the gradient's output, the divergence's input) is aligned for 2-vector accesses
+#define OPY_LAUNCH_2F(KERN, T, in, out, g, w, add, st, vec)                                                      \
+  do {                                                                                                         \
+    const int H_ = (int)g.dims[0], W_ = (int)g.dims[1], C_ = g.nchan;                                          \
+    const dim3 grid_((unsigned)((W_ * C_ + kBlock - 1) / kBlock), (unsigned)H_);                                \
+    if (C_ == 1 && (vec))                                                                                      \
+      hipLaunchKernelGGL((KERN<T, 1>), grid_, dim3(kBlock), 0, st, in, out, H_, W_, C_, w, g.h[0], g.h[1], add); \
+    else if (C_ == 2)                                                                                          \
+      hipLaunchKernelGGL((KERN<T, 2>), grid_, dim3(kBlock), 0, st, in, out, H_, W_, C_, w, g.h[0], g.h[1], add); \
+    else if (C_ == 3)                                                                                          \
+      hipLaunchKernelGGL((KERN<T, 3>), grid_, dim3(kBlock), 0, st, in, out, H_, W_, C_, w, g.h[0], g.h[1], add); \
+    else                                                                                                       \
+      hipLaunchKernelGGL((KERN<T, 0>), grid_, dim3(kBlock), 0, st, in, out, H_, W_, C_, w, g.h[0], g.h[1], add); \
+  } while (0)
+
 // Row-wise l^p norms |A_i|_p^p of the gradient (rows = (point, t, channel)).
 template <typename T, bool kCentered>
 __global__ void __launch_bounds__(kBlock) grad_rowwise_kernel(T* __restrict__ y, Grid g,
@@ -357,7 +456,10 @@ int grad_impl(const T* x, T* y, const opy_grid_t* gd, const double* w, int schem
   const int64_t n = g.npoints * g.nchan;
   if (scheme == OPY_SCHEME_CENTERED)
     hipLaunchKernelGGL((grad_kernel<T, true>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), x, y, g, w, add);
-  else if (scheme == OPY_SCHEME_FORWARD)
+  else if (scheme == OPY_SCHEME_FORWARD && use_2d_forward(g)) {
+    if (n > 0) OPY_LAUNCH_2F(grad2f_kernel, T, x, y, g, w, add, as_stream(s),
+                             reinterpret_cast<uintptr_t>(y) % (2 * sizeof(T)) == 0);
+  } else if (scheme == OPY_SCHEME_FORWARD)
     hipLaunchKernelGGL((grad_kernel<T, false>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), x, y, g, w, add);
   else
     return fail(kInvalid, "gradient: unknown scheme %d", scheme);
@@ -372,7 +474,10 @@ int div_impl(const T* p, T* x, const opy_grid_t* gd, const double* w, int scheme
   const int64_t n = g.npoints * g.nchan;
   if (scheme == OPY_SCHEME_CENTERED)
     hipLaunchKernelGGL((div_kernel<T, true>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), p, x, g, w, add);
-  else if (scheme == OPY_SCHEME_FORWARD)
+  else if (scheme == OPY_SCHEME_FORWARD && use_2d_forward(g)) {
+    if (n > 0) OPY_LAUNCH_2F(div2f_kernel, T, p, x, g, w, add, as_stream(s),
+                             reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T)) == 0);
+  } else if (scheme == OPY_SCHEME_FORWARD)
     hipLaunchKernelGGL((div_kernel<T, false>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), p, x, g, w, add);
   else
     return fail(kInvalid, "divergence: unknown scheme %d", scheme);

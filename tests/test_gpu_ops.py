@@ -60,6 +60,37 @@ def test_gradient_divergence(gpu, dtype, dims, C, scheme):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("C,offset", [(1, 1), (1, 0), (4, 0), (2, 3)])
+def test_gradient_divergence_2d_views(gpu, dtype, C, offset):
+    """The 2-D forward kernels on views at element offsets (the 2-vector
+    accesses of C = 1 need an aligned field, else the scalar form runs), for
+    C = 1, 2 and 4, with and without add, against the oracle matrices."""
+    from opymize_amd.linear import GradientOp
+    import torch
+    rng = np.random.default_rng(7)
+    dims = (37, 45)
+    h = rng.uniform(0.5, 2, 2)
+    w = rng.uniform(0.5, 2, C)
+    op = GradientOp(dims, C, scheme="forward", imageh=h, weights=w)
+    G = ops.grad_matrix(dims, C, h, w, "forward")
+    x = rng.standard_normal(op.x.size)
+    p = rng.standard_normal(op.y.size)
+    xr = x.astype(dtype).astype(np.float64)
+    pr = p.astype(dtype).astype(np.float64)
+    for add in (False, True):
+        yb = dev(np.concatenate([np.zeros(offset), rng.standard_normal(op.y.size)]), dtype)
+        y0 = host(yb)[offset:]
+        xb = dev(np.concatenate([np.zeros(offset), x]), dtype)
+        op(xb[offset:], yb[offset:], add=add)
+        close(host(yb)[offset:], (y0 if add else 0) + G.dot(xr), dtype)
+        pb = dev(np.concatenate([np.zeros(offset), p]), dtype)
+        ob = dev(np.concatenate([np.zeros(offset), rng.standard_normal(op.x.size)]), dtype)
+        o0 = host(ob)[offset:]
+        op.adjoint(pb[offset:], ob[offset:], add=add)
+        close(host(ob)[offset:], (o0 if add else 0) + G.T.dot(pr), dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("bc", ["neumann", "curvature", "curvature_adj", "second-order", "second-order_adj"])
 def test_laplacian(gpu, dtype, bc):
     from opymize_amd.linear import LaplacianOp
