@@ -200,6 +200,79 @@ __global__ void __launch_bounds__(kBlock) div2f_kernel(const T* __restrict__ p, 
   *out = add ? *out + val : val;
 }
 
+// The same on a 3-D grid (d0 x d1 x d2): a thread per (i1, i2, channel) of
+// slab i0 = blockIdx.y, with 32-bit index arithmetic (the lifting model's
+// (H, W, labels) grid, whose label axis is short).
+template <typename T, int CT>
+__global__ void __launch_bounds__(kBlock) grad3f_kernel(const T* __restrict__ x, T* __restrict__ y, int d0, int d1,
+                                                        int d2, int Cr, const double* __restrict__ w, double h0,
+                                                        double h1, double h2, int add) {
+  const int C = CT > 0 ? CT : Cr;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = d2 * C;
+  if (e >= d1 * row) return;
+  const int i0 = blockIdx.y;
+  const int i1 = e / row;
+  const int rem = e - i1 * row;
+  const int i2 = rem / C;
+  const int k = rem - i2 * C;
+  const int i = (i0 * d1 + i1) * d2 + i2;
+  const double wk = w ? w[k] : 1.0;
+  const T x0 = x[i * C + k];
+  T v0 = T(0), v1 = T(0), v2 = T(0);
+  if (i0 < d0 - 1) v0 = (T)(wk / h0) * (x[(i + d1 * d2) * C + k] - x0);
+  if (i1 < d1 - 1) v1 = (T)(wk / h1) * (x[(i + d2) * C + k] - x0);
+  if (i2 < d2 - 1) v2 = (T)(wk / h2) * (x[(i + 1) * C + k] - x0);
+  T* out = y + (i * 3) * C + k;
+  out[0] = add ? out[0] + v0 : v0;
+  out[C] = add ? out[C] + v1 : v1;
+  out[2 * C] = add ? out[2 * C] + v2 : v2;
+}
+
+template <typename T, int CT>
+__global__ void __launch_bounds__(kBlock) div3f_kernel(const T* __restrict__ p, T* __restrict__ x, int d0, int d1,
+                                                       int d2, int Cr, const double* __restrict__ w, double h0,
+                                                       double h1, double h2, int add) {
+  const int C = CT > 0 ? CT : Cr;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = d2 * C;
+  if (e >= d1 * row) return;
+  const int i0 = blockIdx.y;
+  const int i1 = e / row;
+  const int rem = e - i1 * row;
+  const int i2 = rem / C;
+  const int k = rem - i2 * C;
+  const int i = (i0 * d1 + i1) * d2 + i2;
+  const double wk = w ? w[k] : 1.0;
+  const T ih0 = (T)(1.0 / h0), ih1 = (T)(1.0 / h1), ih2 = (T)(1.0 / h2);
+  T acc = T(0);
+  if (i0 < d0 - 1) acc -= p[(i * 3) * C + k] * ih0;
+  if (i0 > 0) acc += p[((i - d1 * d2) * 3) * C + k] * ih0;
+  if (i1 < d1 - 1) acc -= p[(i * 3 + 1) * C + k] * ih1;
+  if (i1 > 0) acc += p[((i - d2) * 3 + 1) * C + k] * ih1;
+  if (i2 < d2 - 1) acc -= p[(i * 3 + 2) * C + k] * ih2;
+  if (i2 > 0) acc += p[((i - 1) * 3 + 2) * C + k] * ih2;
+  const T val = (T)wk * acc;
+  T* out = x + i * C + k;
+  *out = add ? *out + val : val;
+}
+
+inline bool use_3d_forward(const Grid& g) {
+  return g.ndims == 3 && g.dims[0] <= 65535 && g.npoints * g.nchan * 3 < (int64_t(1) << 31);
+}
+
+#define OPY_LAUNCH_3F(KERN, T, in, out, g, w, add, st)                                                        \
+  do {                                                                                                      \
+    const int d0_ = (int)g.dims[0], d1_ = (int)g.dims[1], d2_ = (int)g.dims[2], C_ = g.nchan;                \
+    const dim3 grid_((unsigned)((d1_ * d2_ * C_ + kBlock - 1) / kBlock), (unsigned)d0_);                     \
+    if (C_ == 1)                                                                                            \
+      hipLaunchKernelGGL((KERN<T, 1>), grid_, dim3(kBlock), 0, st, in, out, d0_, d1_, d2_, C_, w, g.h[0],     \
+                         g.h[1], g.h[2], add);                                                              \
+    else                                                                                                    \
+      hipLaunchKernelGGL((KERN<T, 0>), grid_, dim3(kBlock), 0, st, in, out, d0_, d1_, d2_, C_, w, g.h[0],     \
+                         g.h[1], g.h[2], add);                                                              \
+  } while (0)
+
 // The 2-D kernels serve forward 2-D grids whose indices fit 32 bits and whose
 // rows fit the grid's second dimension.
 inline bool use_2d_forward(const Grid& g) {
@@ -459,6 +532,8 @@ int grad_impl(const T* x, T* y, const opy_grid_t* gd, const double* w, int schem
   else if (scheme == OPY_SCHEME_FORWARD && use_2d_forward(g)) {
     if (n > 0) OPY_LAUNCH_2F(grad2f_kernel, T, x, y, g, w, add, as_stream(s),
                              reinterpret_cast<uintptr_t>(y) % (2 * sizeof(T)) == 0);
+  } else if (scheme == OPY_SCHEME_FORWARD && use_3d_forward(g)) {
+    if (n > 0) OPY_LAUNCH_3F(grad3f_kernel, T, x, y, g, w, add, as_stream(s));
   } else if (scheme == OPY_SCHEME_FORWARD)
     hipLaunchKernelGGL((grad_kernel<T, false>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), x, y, g, w, add);
   else
@@ -477,6 +552,8 @@ int div_impl(const T* p, T* x, const opy_grid_t* gd, const double* w, int scheme
   else if (scheme == OPY_SCHEME_FORWARD && use_2d_forward(g)) {
     if (n > 0) OPY_LAUNCH_2F(div2f_kernel, T, p, x, g, w, add, as_stream(s),
                              reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T)) == 0);
+  } else if (scheme == OPY_SCHEME_FORWARD && use_3d_forward(g)) {
+    if (n > 0) OPY_LAUNCH_3F(div3f_kernel, T, p, x, g, w, add, as_stream(s));
   } else if (scheme == OPY_SCHEME_FORWARD)
     hipLaunchKernelGGL((div_kernel<T, false>), dim3(grid_for(n)), dim3(kBlock), 0, as_stream(s), p, x, g, w, add);
   else

@@ -29,7 +29,8 @@ def close(a, b, dtype, scale=None):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("dims,C", [((7, 9), 1), ((16, 33), 3), ((5, 6, 7), 2), ((300,), 1), ((1, 17), 2)])
+@pytest.mark.parametrize("dims,C", [((7, 9), 1), ((16, 33), 3), ((5, 6, 7), 2), ((6, 11, 16), 1), ((300,), 1),
+                                    ((1, 17), 2)])
 @pytest.mark.parametrize("scheme", ["forward", "centered"])
 def test_gradient_divergence(gpu, dtype, dims, C, scheme):
     from opymize_amd.linear import GradientOp
