@@ -294,3 +294,47 @@ def test_quasinewton_rounds_match_scipy(gpu):
             for k in ("objp", "objd", "infeasd"):
                 assert abs(c[k] - g[k]) <= 1e-8 * max(1.0, abs(g[k])), (gran, k)
         assert info["relgap"] == s.checks[-1]["relgap"]
+
+
+def test_newton_on_a_split_problem_matches_oracle(gpu):
+    """A problem whose F is a SplitSum and whose A is a BlockOp: TV-L2 with
+    x >= 0 written as F = (tv |.|, indicator(. >= 0)) of (grad x, x).  The
+    dual prox is then a SplitOp of the ball projection and NegProj, and its
+    Jacobian a block-diagonal BlockOp; five Newton steps against the oracle
+    with the same blocks as scipy matrices."""
+    import scipy.sparse as sp
+    from opymize_amd.functionals import SSD, L1Norms, PositivityFct, SplitSum
+    from opymize_amd.linear import BlockOp, GradientOp, IdentityOp
+    from opymize_amd.solvers import SemismoothNewton
+    from oracle import ops, prox
+    H, W, lam, tvw = 9, 8, 3.0, 0.2
+    N = H * W
+    f = rects(H, W, 1, 11) - 0.4  # partly negative data, so that the constraint acts
+    G = SSD(f, vol=np.array([lam]))
+    F = SplitSum([L1Norms(N, (1, 2), lbd=tvw), PositivityFct(N)])
+    A = BlockOp([[GradientOp((H, W), 1, scheme="forward")], [IdentityOp(N)]])
+    rng = np.random.default_rng(12)
+    xy0 = np.concatenate([rng.random(N), 0.1 * rng.standard_normal(2 * N), -0.1 * rng.random(N)])
+    s = SemismoothNewton(G, F, A)
+    s.solve(continue_at=xy0, term_maxiter=5, granularity=1000, term_relgap=0.0)
+    tau, sigma = s.constvars["tau"], s.constvars["sigma"]
+    Ao = sp.vstack([ops.grad_matrix((H, W), 1, None, None, "forward"), sp.eye(N)]).tocsr()
+    lamv = np.array([lam])
+
+    def gprox(x, t):
+        return prox.ssd_prox(x, f, lamv, t), onw.ssd_prox_jacobian(N, 1, lamv, t)
+
+    def fconjprox(y, sg):
+        y1, y2 = y[:2 * N], y[2 * N:]
+        val = np.concatenate([prox.l1norms_proj(y1, N, 1, 2, tvw), np.minimum(y2, 0.0)])
+        jac = sp.block_diag([onw.l12proj_jacobian(y1, N, 2, tvw), onw.signproj_jacobian(y2, False)])
+        return val, jac
+
+    prob = onw.NewtonProblem(Ao, gprox, fconjprox, None, None, None, None)
+    xy, hist = onw.newton_solve(prob, xy0, tau, sigma, 5)
+    for a, b in zip(s.itervars["history"], hist):
+        assert a["alpha"] == b["alpha"] and a["cg_steps"] == b["cg_steps"]
+        assert abs(a["res_normsq"] - b["res_normsq"]) <= 1e-9 * b["res_normsq"]
+    x, y = s.state
+    np.testing.assert_allclose(np.concatenate([x, y]), xy, rtol=0, atol=1e-8)
+    assert (xy[3 * N:] < 0).any() and (xy[N:3 * N] != 0).any()
