@@ -408,12 +408,14 @@ int opy_tv_multi_f64(const double* x_in, double* x_out, const double* y_in, doub
                      const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,
                      const int32_t* act_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int grid,
                      void* stream);
-/* the same k in {2,3,4} iterations in one pass over HBM with steps SCALAR or ACCEL, the same
- * iterates bit for bit, by a different work split: one wavefront per unit of 64-2k image columns by
- * `rows` image rows, walking down the unit and k rows above and below it with the k iterations
- * pipelined one row apart (no LDS).  unit_off (B+1 entries) is the running instances' exclusive
- * prefix of units, from opy_tv_stream_units with the same k and rows; `grid` as above (one block
- * of 4 wavefronts per 4 units streams fastest).  PRECOND steps are refused (use opy_tv_multi).
+/* k in 2..8 iterations (k > 4 for C <= 2) in one pass over HBM with steps SCALAR or ACCEL, the
+ * iterates of opy_tv_multi / k single launches bit for bit, by a different work split: one wavefront
+ * per unit of 64-2k image columns by `rows` image rows, walking down the unit and k rows above and
+ * below it with the k iterations pipelined one row apart (no LDS).  unit_off (B+1 entries) is the
+ * running instances' exclusive prefix of units, from opy_tv_stream_units with the same k and rows;
+ * `grid` as above (one block of 4 wavefronts per 4 units streams fastest).  PRECOND steps are
+ * refused (use opy_tv_multi).  Pixels are addressed by 32-bit byte offsets from each instance's
+ * base, so every instance's y block (H*W*2C values) must be below 4 GiB.
  * No reference counterpart beyond the iteration itself (PDHG.iteration_step, pdhg.py:124-153). */
 int opy_tv_stream_f32(const float* x_in, float* x_out, const float* y_in, float* y_out, const float* f,
                       const opy_tv_inst_t* insts, const opy_adapt_state_t* steps, const int32_t* active_inst,

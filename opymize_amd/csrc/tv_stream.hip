@@ -58,6 +58,16 @@ __device__ inline FoldedStep<T, C> bcast_step(const FoldedStep<T, C>& s, int l) 
 
 constexpr int kStreamD = 2;  // rows loaded ahead of the pipeline
 
+// p + a 32-bit byte offset: a uniform base and a zero-extended vector offset
+template <typename T>
+__device__ inline T* byte_off(T* p, uint32_t bytes) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (size_t)bytes);
+}
+template <typename T>
+__device__ inline const T* byte_off(const T* p, uint32_t bytes) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (size_t)bytes);
+}
+
 // plain vector stores (the rows of adjacent strips share cache lines, which
 // the L2 merges; measured no slower than streaming stores here)
 template <typename T, int N>
@@ -137,10 +147,10 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
 #pragma unroll
         for (int e = 0; e < 2 * C; ++e) y[e] = T(0);
         if (!M || (r >= 0 && r < H && c >= 0 && c < W)) {
-          const int64_t p = (int64_t)r * W + c;
-          load_c<T, 2 * C>(yb + p * 2 * C, y);
-          load_c<T, C>(xb + p * C, x);
-          load_c<T, C>(fb + p * C, fv);
+          const uint32_t p = (uint32_t)(r * W) + (uint32_t)c;
+          load_c<T, 2 * C>(byte_off(yb, p * (uint32_t)(2 * C * sizeof(T))), y);
+          load_c<T, C>(byte_off(xb, p * (uint32_t)(C * sizeof(T))), x);
+          load_c<T, C>(byte_off(fb, p * (uint32_t)(C * sizeof(T))), fv);
         }
       };
 #pragma unroll
@@ -202,9 +212,9 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
         const int m = n - K;
         const int r = r0 + m;
         if (m >= K && r < H && lane >= K && lane < 64 - K && c < W) {
-          const int64_t p = (int64_t)r * W + c;
-          store_v<T, C>(xob + p * C, xcur);
-          store_v<T, 2 * C>(yob + p * 2 * C, ycur);
+          const uint32_t p = (uint32_t)(r * W) + (uint32_t)c;
+          store_v<T, C>(byte_off(xob, p * (uint32_t)(C * sizeof(T))), xcur);
+          store_v<T, 2 * C>(byte_off(yob, p * (uint32_t)(2 * C * sizeof(T))), ycur);
         }
       }
     };

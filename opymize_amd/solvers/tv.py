@@ -254,7 +254,19 @@ class TVBatch(object):
                 raise ValueError("the streaming kernel has no preconditioned steps; use multi_kernel='tile' "
                                  "(at most 4 iterations per launch)")
             return False
+        if not self._stream_fits():
+            if self.multi_kernel == "stream" or self.multi > 4:
+                raise ValueError("the streaming kernel addresses an instance's y with 32-bit byte offsets; the "
+                                 "largest instance here needs more: use multi_kernel='tile' (at most 4 "
+                                 "iterations per launch)")
+            return False
         return self.multi_kernel == "stream" or self._stream_now
+
+    def _stream_fits(self):
+        """Every instance's y block (2 C values per pixel) is below 4 GiB, the
+        reach of the streaming kernel's 32-bit offsets."""
+        esize = 4 if self.dtype == torch.float32 else 8
+        return max(H * W for (H, W) in self.shapes) * 2 * self.C * esize < (1 << 32)
 
     def _launch_k(self, cfg):
         """Iterations per K-iteration launch: multi, or at most 4 on the tile

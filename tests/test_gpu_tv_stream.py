@@ -254,6 +254,17 @@ def test_stream_rejects_bad_arguments(gpu):
         TVBatch([np.zeros((8, 8))], 1.0, multi=2, multi_kernel="lds")
     with pytest.raises(ValueError):
         TVBatch([np.zeros((8, 8))], 1.0, multi=2, stream_rows=0)
+    # an instance whose y block would pass 4 GiB (32-bit byte offsets): tiles under "auto",
+    # refused when streaming is forced (the shapes are stood in, nothing is launched)
+    big = TVBatch([np.zeros((8, 8))], 1.0, multi=4)
+    cfg = big._cfg("accel")
+    big._stream_now = True
+    assert big._use_stream(cfg)
+    big.shapes = [(23171, 23171)]  # 2 x 4 B x 23171^2 > 2^32
+    assert not big._stream_fits() and not big._use_stream(cfg)
+    big.multi_kernel = "stream"
+    with pytest.raises(ValueError, match="32-bit"):
+        big._use_stream(cfg)
     bat, *_ = _batch("float", 4, "stream")
     with pytest.raises(ValueError, match="preconditioned"):
         bat.solve(steps="precond", step_bound=0.124, max_iters=8, block_iters=8, tol=1e-30)
