@@ -416,14 +416,16 @@ __device__ inline FoldedStep<T, C> fold_step(double tau, double sigma, double th
 
 // x+ = prox_{tau G}(x - tau A^T y) at pixel (r, c), channel k, from x, f and
 // y = (y0, y1) there, y0 of the pixel above and y1 of the pixel to the left
-// (the terms adjoint_px masks at the image boundary are dropped by selects
-// when M); L1: the data kind
-template <typename T, int C, bool L1, bool M>
-__device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int c, int H, int W, int k,
-                              const FoldedStep<T, C>& s) {
-  if (M) {
+// (the terms adjoint_px masks at the image boundary are dropped by selects:
+// the row tests when MR, the column tests when MC); L1: the data kind
+template <typename T, int C, bool L1, bool MR, bool MC>
+__device__ inline T primal_px_rc(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int c, int H, int W, int k,
+                                 const FoldedStep<T, C>& s) {
+  if (MR) {
     if (!(r < H - 1)) y0 = T(0);
     if (!(r > 0)) yup0 = T(0);
+  }
+  if (MC) {
     if (!(c < W - 1)) y1 = T(0);
     if (!(c > 0)) yleft1 = T(0);
   }
@@ -435,17 +437,23 @@ __device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int
     return mul_rn(s.a, v + s.b * f);
   }
 }
+template <typename T, int C, bool L1, bool M>
+__device__ inline T primal_px(T x, T f, T y0, T y1, T yup0, T yleft1, int r, int c, int H, int W, int k,
+                              const FoldedStep<T, C>& s) {
+  return primal_px_rc<T, C, L1, M, M>(x, f, y0, y1, yup0, yleft1, r, c, H, W, k, s);
+}
 
 // y+ = proj_{|.| <= tvw}(y + sigma A xbar) at pixel (r, c) from xbar there,
-// below and to the right (differences beyond the image are 0 when M)
-template <typename T, int C, bool M>
-__device__ inline void dual_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
-                               const FoldedStep<T, C>& s, T tvw, T* yp) {
+// below and to the right (differences beyond the image are 0: down when MR,
+// right when MC)
+template <typename T, int C, bool MR, bool MC>
+__device__ inline void dual_px_rc(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
+                                  const FoldedStep<T, C>& s, T tvw, T* yp) {
 #pragma unroll
   for (int k = 0; k < C; ++k) {
     T gd = xbd[k] - xb[k], gr = xbr[k] - xb[k];
-    if (M && !(r < H - 1)) gd = T(0);
-    if (M && !(c < W - 1)) gr = T(0);
+    if (MR && !(r < H - 1)) gd = T(0);
+    if (MC && !(c < W - 1)) gr = T(0);
     yp[k] = y[k] + gd * s.q[0][k];
     yp[C + k] = y[C + k] + gr * s.q[1][k];
   }
@@ -461,6 +469,11 @@ __device__ inline void dual_px(const T* xb, const T* xbd, const T* xbr, const T*
 #pragma unroll
     for (int e = 0; e < 2 * C; ++e) yp[e] = mul_rn(yp[e], sc);
   }
+}
+template <typename T, int C, bool M>
+__device__ inline void dual_px(const T* xb, const T* xbd, const T* xbr, const T* y, int r, int c, int H, int W,
+                               const FoldedStep<T, C>& s, T tvw, T* yp) {
+  dual_px_rc<T, C, M, M>(xb, xbd, xbr, y, r, c, H, W, s, tvw, yp);
 }
 
 // ---- one pass per iteration -------------------------------------------------

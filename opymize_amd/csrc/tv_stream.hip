@@ -120,15 +120,16 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
       for (int i = 0; i < K; ++i) fs[i] = bcast_step(mine, i);
     }
     const T tvw = (T)I.tvw;
-    const bool inner = r0 >= 1 && r0 + NSTEP < H && c0 >= 1 && c0 + 64 < W;
+    // the unit's columns reach past the image (the first and last strips)
+    const bool cedge = !(c0 >= 1 && c0 + 64 < W);
     const T* xb = xin + I.off;
     const T* fb = f + I.off;
     const T* yb = yin + 2 * I.off;
     T* xob = xout + I.off;
     T* yob = yout + 2 * I.off;
-    // L1: data kind, M: boundary masks and bounds tests (both uniform over the unit)
-    auto run = [&](auto l1_tag, auto m_tag) {
-      constexpr bool L1 = decltype(l1_tag)::value, M = decltype(m_tag)::value;
+    // L1: data kind; MC: column masks and column bounds tests (uniform over the unit)
+    auto run = [&](auto l1_tag, auto mc_tag) {
+      constexpr bool L1 = decltype(l1_tag)::value, MC = decltype(mc_tag)::value;
       // per level: x^(i+1) and x_bar^(i+1) of its last row, y^i of the row
       // above its current one, f delayed by i rows
       T Xo[K][C], XB[K][C], YL[K][2 * C], F[K][C];
@@ -140,23 +141,25 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
         for (int e = 0; e < 2 * C; ++e) YL[i][e] = T(0);
       }
       T nx[D][C], nf[D][C], ny[D][2 * C];
-      auto load_row = [&](int n, T* x, T* fv, T* y) {
+      // LB: bounds tests (the row's, uniform, and the lane's column when MC)
+      auto load_row = [&](auto lb_tag, int n, T* x, T* fv, T* y) {
+        constexpr bool LB = decltype(lb_tag)::value;
         const int r = r0 + n;
-#pragma unroll
-        for (int k = 0; k < C; ++k) x[k] = fv[k] = T(0);
-#pragma unroll
-        for (int e = 0; e < 2 * C; ++e) y[e] = T(0);
-        if (!M || (r >= 0 && r < H && c >= 0 && c < W)) {
+        if (!LB || (r >= 0 && r < H && (!MC || (c >= 0 && c < W)))) {
           const uint32_t p = (uint32_t)(r * W) + (uint32_t)c;
           load_c<T, 2 * C>(byte_off(yb, p * (uint32_t)(2 * C * sizeof(T))), y);
           load_c<T, C>(byte_off(xb, p * (uint32_t)(C * sizeof(T))), x);
           load_c<T, C>(byte_off(fb, p * (uint32_t)(C * sizeof(T))), fv);
+        } else {
+#pragma unroll
+          for (int k = 0; k < C; ++k) x[k] = fv[k] = T(0);
+#pragma unroll
+          for (int e = 0; e < 2 * C; ++e) y[e] = T(0);
         }
       };
-#pragma unroll
-      for (int d = 0; d < D; ++d) load_row(d, nx[d], nf[d], ny[d]);
-#pragma unroll 1
-      for (int n = 0; n < NSTEP; ++n) {
+      // one pipeline step; MR: row masks and row bounds tests of the load
+      auto step = [&](auto mr_tag, int n) {
+        constexpr bool MR = decltype(mr_tag)::value;
         T xcur[C], ycur[2 * C];
 #pragma unroll
         for (int i = K - 1; i > 0; --i)
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
 #pragma unroll
           for (int e = 0; e < 2 * C; ++e) ny[d][e] = ny[d + 1][e];
         }
-        if (n + D < NSTEP) load_row(n + D, nx[D - 1], nf[D - 1], ny[D - 1]);
+        if (n + D < NSTEP) load_row(mr_tag, n + D, nx[D - 1], nf[D - 1], ny[D - 1]);
 #pragma unroll
         for (int i = 0; i < K; ++i) {
           const int rx = r0 + n - i, ry = rx - 1;  // rows of x^(i+1) and y^(i+1) of this step
@@ -189,12 +192,12 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
 #pragma unroll
           for (int k = 0; k < C; ++k) {
             const T yl1 = dpp_shr1(T(0), ycur[C + k]);  // y1 of the left neighbour
-            xnew[k] = primal_px<T, C, L1, M>(xcur[k], F[i][k], ycur[k], ycur[C + k], YL[i][k], yl1, rx, c, H, W, k,
-                                             s);
+            xnew[k] = primal_px_rc<T, C, L1, MR, MC>(xcur[k], F[i][k], ycur[k], ycur[C + k], YL[i][k], yl1, rx, c,
+                                                     H, W, k, s);
             xbar[k] = overrelax(theta1, s.theta, xnew[k], xcur[k]);
             xbr[k] = dpp_shl1(T(0), XB[i][k]);  // x_bar of the right neighbour, row ry
           }
-          dual_px<T, C, M>(XB[i], xbar, xbr, YL[i], ry, c, H, W, s, tvw, yp);
+          dual_px_rc<T, C, MR, MC>(XB[i], xbar, xbr, YL[i], ry, c, H, W, s, tvw, yp);
 #pragma unroll
           for (int e = 0; e < 2 * C; ++e) {
             YL[i][e] = ycur[e];
@@ -216,16 +219,40 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
           store_v<T, C>(byte_off(xob, p * (uint32_t)(C * sizeof(T))), xcur);
           store_v<T, 2 * C>(byte_off(yob, p * (uint32_t)(2 * C * sizeof(T))), ycur);
         }
+      };
+      using TT = std::true_type;
+      using FF = std::false_type;
+#pragma unroll
+      for (int d = 0; d < D; ++d) load_row(TT{}, d, nx[d], nf[d], ny[d]);
+      // Steps whose rows (every level's, and the row loaded) lie inside the
+      // image with a row to spare run without row masks and row bounds
+      // tests: steps [nA, nB).  Level i's rows are r0 + n - i and the one
+      // above, so n >= K - r0 and n <= H - 2 - r0; the row loaded is
+      // r0 + n + D when n + D < NSTEP.  Units away from the top and bottom
+      // edges run every step there.
+      int nA = NSTEP, nB = NSTEP;
+      if (!MC) {
+        nA = min(NSTEP, max(0, K - r0));
+        nB = min(NSTEP, H - 1 - r0);
+        if (r0 + NSTEP > H) nB = min(nB, H - r0 - D);
+        nB = max(nA, nB);
       }
+      int n = 0;
+#pragma unroll 1
+      for (; n < nA; ++n) step(TT{}, n);
+#pragma unroll 1
+      for (; n < nB; ++n) step(FF{}, n);
+#pragma unroll 1
+      for (; n < NSTEP; ++n) step(TT{}, n);
     };
     using TT = std::true_type;
     using FF = std::false_type;
     if (I.data == OPY_TV_DATA_L1) {
-      if (inner) run(TT{}, FF{});
-      else run(TT{}, TT{});
+      if (cedge) run(TT{}, TT{});
+      else run(TT{}, FF{});
     } else {
-      if (inner) run(FF{}, FF{});
-      else run(FF{}, TT{});
+      if (cedge) run(FF{}, TT{});
+      else run(FF{}, FF{});
     }
   }
 }
