@@ -28,7 +28,7 @@ Printed next to it:
     instances one at a time through the batched solver's own kernels and
     step rule (so that ratio isolates the gain of batching);
   * the roofline of the dominant kernel (by default the streaming kernel
-    that runs K = 6 iterations per launch, one wavefront per strip of 52
+    that runs K = 8 iterations per launch, one wavefront per strip of 48
     columns by 128 rows, the iterations pipelined down the rows; --multi-kernel
     tile: the LDS-tile kernel, K <= 4; the single-pass iteration kernel with
     --multi 1 or adaptive steps; the dual pass with --passes 2): algorithmic
@@ -525,7 +525,7 @@ def main():
     ap.add_argument("--max-iters", type=int, default=10000)
     ap.add_argument("--block-iters", type=int, default=48)
     ap.add_argument("--tile", type=int, nargs=2, default=[64, 6])
-    ap.add_argument("--multi", type=int, default=6, choices=[1, 2, 3, 4, 5, 6, 7, 8],
+    ap.add_argument("--multi", type=int, default=8, choices=[1, 2, 3, 4, 5, 6, 7, 8],
                     help="iterations per launch (accelerated steps; adaptive steps take 1)")
     ap.add_argument("--passes", type=int, default=1, choices=[1, 2])
     ap.add_argument("--multi-kernel", default="auto", choices=["auto", "stream", "tile"],
