@@ -73,6 +73,25 @@ def test_stream_equals_single_launches(gpu, precision, multi):
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("multi", [6, 8])
+@pytest.mark.parametrize("rows", [8, 16, 128])
+def test_stream_row_phases_equal_single_launches(gpu, precision, multi, rows):
+    """Units whose columns lie inside the image run their interior steps
+    unmasked and only the steps at the top and bottom image rows with row
+    masks: images wide enough for such strips, from 5 rows (no unmasked
+    step) to 200 (units away from both edges), give the same bits as
+    single-iteration launches."""
+    shapes = [(200, 300), (5, 300), (21, 260), (130, 130), (17, 700)]
+    out = {}
+    for m in (1, multi):
+        bat, *_ = _batch(precision, m, "stream", shapes=shapes, seed=13, rows=rows)
+        (xs, ys), _ = _solve(bat, "accel", 48, 24)
+        out[m] = xs + ys
+    for a, b in zip(out[1], out[multi]):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("C", [2, 3, 4])
 def test_stream_channels(gpu, precision, C):
     """Per-pixel vectors of C and 2C values (Frobenius-norm TV): the streaming
