@@ -92,6 +92,22 @@ def test_stream_row_phases_equal_single_launches(gpu, precision, multi, rows):
 
 
 @pytest.mark.parametrize("precision", ["double", "float"])
+@pytest.mark.parametrize("multi", [6, 8])
+def test_stream_two_channels_equal_single_launches(gpu, precision, multi):
+    """Two channels (the largest C built beyond 4 iterations per launch),
+    TV-L2 and TV-L1, interior strips and row phases: the same bits as
+    single-iteration launches."""
+    shapes = [(40, 300), (9, 200), (70, 41), (150, 160)]
+    out = {}
+    for m in (1, multi):
+        bat, *_ = _batch(precision, m, "stream", shapes=shapes, C=2, seed=17, rows=16)
+        (xs, ys), _ = _solve(bat, "accel", 48, 24)
+        out[m] = xs + ys
+    for a, b in zip(out[1], out[multi]):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
 @pytest.mark.parametrize("C", [2, 3, 4])
 def test_stream_channels(gpu, precision, C):
     """Per-pixel vectors of C and 2C values (Frobenius-norm TV): the streaming
