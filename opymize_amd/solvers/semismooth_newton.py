@@ -74,7 +74,7 @@ def armijo(fun, xk, xkp1, p, p_gradf, fun_xk, eta=0.5, nu=0.9):
     fun(xk + alpha p) <= fun(xk) + alpha nu p_gradf; sets xkp1 = xk + alpha p
     and returns (fun(xkp1), alpha) (reference: semismooth_newton.py:14-47)."""
     if p_gradf >= 0:
-        raise Exception("Armijo: Not a descent direction!")
+        raise Exception("Armijo: the direction does not descend (slope %g)" % p_gradf)
     alpha = 1.0
     while True:
         _axpby(xk, p, xkp1, 1.0, alpha)
@@ -83,7 +83,7 @@ def armijo(fun, xk, xkp1, p, p_gradf, fun_xk, eta=0.5, nu=0.9):
             break
         alpha *= eta
         if alpha < 1e-10:
-            raise Exception("Armijo: Not a descent direction!")
+            raise Exception("Armijo: no sufficient decrease down to step %g" % alpha)
     return f_xkp1, alpha
 
 
@@ -480,8 +480,7 @@ def broyden1(F, x0, iters):
             G.matvec(Fx, dx)
         _axpby(dx, dx, dx, -1.0, 0.0)  # dx = -G F(x)
         if _norm(dx) == 0:
-            raise ValueError("Jacobian inversion yielded zero vector. This indicates a bug in the Jacobian "
-                             "approximation.")
+            raise ValueError("broyden1: the inverse Jacobian estimate maps F(x) to zero")
         # line search on phi(s) = |F(x + s dx)|^2 (scipy _nonlin_line_search, 'armijo')
         store = {"s": 0.0, "phi": _norm(Fx) ** 2, "F": Fx}
 
