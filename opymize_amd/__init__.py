@@ -98,7 +98,7 @@ class Operator(object):
         _hip.torch_dtype(type_t)
 
     def _call_gpu(self, x, y=None, add=False, jacobian=False):
-        raise Exception("This operator does not support GPU arrays")
+        raise Exception("%s has no device implementation" % type(self).__name__)
 
     def __call__(self, x, y=None, add=False, jacobian=False):
         """Evaluate T(x); written to x unless y is given; add accumulates."""
@@ -118,7 +118,7 @@ class LinOp(Operator):
         self.adjoint = None
 
     def _call_gpu(self, x, y=None, add=False):
-        raise Exception("This operator does not support GPU arrays")
+        raise Exception("%s has no device implementation" % type(self).__name__)
 
     def __call__(self, x, y=None, add=False, jacobian=False):
         if _is_dev(x):
@@ -130,7 +130,7 @@ class LinOp(Operator):
         return self if jacobian else None
 
     def _rowwise_gpu(self, y, p=1, add=False):
-        raise Exception("This operator does not support rowwise l^p norms.")
+        raise Exception("%s has no row-wise l^p norms" % type(self).__name__)
 
     def rowwise_lp(self, y, p=1, add=False):
         """y_i (+)= sum_j |A_ij|^p."""
@@ -176,10 +176,10 @@ class Functional(object):
         return [(0, self.x.size // w, w)]
 
     def _gpu_eval(self, x, out3):
-        raise Exception("This functional does not support being called")
+        raise Exception("%s cannot be evaluated" % type(self).__name__)
 
     def _gpu_finish(self, v):
-        raise Exception("This functional does not support being called")
+        raise Exception("%s cannot be evaluated" % type(self).__name__)
 
     def __call__(self, x, grad=False):
         if grad:
@@ -195,4 +195,4 @@ class Functional(object):
         return self._gpu_finish(out.cpu().numpy())
 
     def prox(self, tau):
-        raise Exception("This functional does not support taking prox")
+        raise Exception("%s has no prox operator" % type(self).__name__)

@@ -125,7 +125,7 @@ class LaplacianOp(LinOp):
             else:
                 self.adjoint = adjoint
         else:
-            raise Exception("Unknown boundary conditions: %s" % self.bc)
+            raise Exception("LaplacianOp: boundary condition %r is not one of %s" % (self.bc, list(self.supported_bc)))
 
     def _code(self):
         return ord(self.bc[0].upper()), int(self.bc.endswith("_adj"))
