@@ -544,7 +544,7 @@ int opy_conv3x3_first(const float* f, int64_t stride, const float* loglam, void*
                       const float* bias, int N, int H, int W, int grid, void* stream);
 /* The last layer into the packed solver start (no counterpart): with o = conv(x) + b (BF16 NHWC
  * input, no activation), x_out = ((f - 0.5) + 0.5) + o0 and y_out = (o1, o2) per pixel, image n at
- * f / x_out + n*stride and y_out + 2n*stride. */
+ * f / x_out + n*stride and y_out + 2n*stride; x 16-byte and y_out 8-byte aligned. */
 int opy_conv3x3_last(const void* x, const float* f, int64_t stride, float* x_out, float* y_out, const void* wt,
                      const float* bias, int N, int H, int W, int grid, void* stream);
 int opy_conv3x3_wgrad_bf16(const float* x, const float* g, float* partials, int N, int H, int W, int grid,

@@ -262,24 +262,33 @@ __global__ void __launch_bounds__(256) conv3x3_pipe_kernel(const __bf16* __restr
     int n, h0, w0;
     origin(t, n, h0, w0);
     if constexpr (MODE == kLast) {
-      const float* fb = io.f + (int64_t)n * io.stride;
-      float* xo = io.xo + (int64_t)n * io.stride;
-      float* yo = io.yo + 2 * (int64_t)n * io.stride;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int h = h0 + wave * 4 + m;
-        if (h >= H || li >= 3) continue;
+      // the 3 output channels of the tile go through LDS (FP32, channel-major,
+      // in the input tile's space) so that each thread decodes one pixel: x
+      // and the y pair leave as 4- and 8-byte stores of consecutive pixels
+      static_assert(3 * kTile * kTile * sizeof(float) <= sizeof(xs), "output staging fits the input tile");
+      float* os = reinterpret_cast<float*>(xs);
+      __syncthreads();  // every wave is done with the input tile
+      if (li < 3) {
         const float bv = bias ? bias[li] : 0.f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int w = w0 + 4 * lk + r;
-          if (w >= W) continue;
-          float v = acc[m][0][r] + bv;
-          if (act == 1) v = v > 0.f ? v : 0.f;
-          const int64_t p = (int64_t)h * W + w;
-          if (li == 0) xo[p] = ((fb[p] - 0.5f) + 0.5f) + v;
-          else yo[2 * p + li - 1] = v;
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float v = acc[m][0][r] + bv;
+            if (act == 1) v = v > 0.f ? v : 0.f;
+            os[li * kTile * kTile + (wave * 4 + m) * kTile + 4 * lk + r] = v;
+          }
         }
+      }
+      __syncthreads();
+      const int h = h0 + tid / kTile, w = w0 + tid % kTile;
+      if (h < H && w < W) {
+        const int64_t p = (int64_t)h * W + w;
+        const float* fb = io.f + (int64_t)n * io.stride;
+        float* xo = io.xo + (int64_t)n * io.stride;
+        float* yo = io.yo + 2 * (int64_t)n * io.stride;
+        xo[p] = ((fb[p] - 0.5f) + 0.5f) + os[tid];
+        *reinterpret_cast<float2*>(yo + 2 * p) = make_float2(os[kTile * kTile + tid], os[2 * kTile * kTile + tid]);
       }
     } else {
       __syncthreads();  // every wave is done with the input tile
@@ -545,6 +554,7 @@ OPY_API int opy_conv3x3_last(const void* x, const float* f, int64_t stride, floa
   if (!x || !f || !xo || !yo || !wt || N < 1 || H < 1 || W < 1 || grid < 1 || stride < (int64_t)H * W)
     return fail(kInvalid, "conv3x3_last: bad arguments");
   if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(kInvalid, "conv3x3_last: input must be 16-byte aligned");
+  if ((reinterpret_cast<uintptr_t>(yo) & 7) != 0) return fail(kInvalid, "conv3x3_last: yo must be 8-byte aligned");
   ConvIO io{f, stride, nullptr, xo, yo};
   hipLaunchKernelGGL((conv3x3_pipe_kernel<kLast>), dim3(grid), dim3(256), 0, as_stream(s),
                      static_cast<const __bf16*>(x), nullptr, static_cast<const __bf16*>(wt), bias, N, H, W, 0, io);
