@@ -729,9 +729,11 @@ def main():
         # beyond 4 the launch is bound by instruction issue, not HBM, and it is
         # the time per iteration the solve needs
         a4 = kbytes * npx / (kt["multi_k4"] * 1e-3) / 1e9
-        line["roofline"]["note"] = ("%d iterations per launch: bound by instruction issue (VALU), %.3f ms per "
-                                    "iteration; the same kernel at 4 per launch is HBM-bound, see roofline_k4"
-                                    % (k_launch, kms / k_launch))
+        line["roofline"]["note"] = ("%d iterations per launch: bound by instruction issue (VALU; SQ_INSTS_VALU in "
+                                    "the committed profile of this kernel), %.3f ms per iteration, %.1f TB/s of "
+                                    "single-iteration traffic (28 B/pixel per iteration); the same kernel at 4 per "
+                                    "launch is HBM-bound, see roofline_k4"
+                                    % (k_launch, kms / k_launch, achieved * k_launch / 1e3))
         line["roofline_k4"] = {"bound": "hbm", "achieved": a4, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": a4 / HBM_PEAK_GBS, "avg_ms": kt["multi_k4"],
                                "ms_per_iteration": kt["multi_k4"] / 4, "blocks_per_launch": kt["grid_k4"],
