@@ -8,9 +8,11 @@ namespace {
 
 // Work unit: one wavefront per (strip, chunk) of an instance.  A strip is
 // 64 - 2K image columns (the exact part), a chunk R image rows.  The 64
-// lanes hold the strip and K columns on each side, one column per lane; the
-// wavefront walks down the chunk and K rows above and below it (R + 2K
-// rows), loading each row of x, f and y once.
+// lanes hold the strip and K columns on each side, one column per lane (the
+// first and last strips of an instance hold K more exact columns instead of
+// the halo beyond the image edge: stream_strips); the wavefront walks down
+// the chunk and K rows above and below it (R + 2K rows), loading each row of
+// x, f and y once.
 //
 // The K iterations run as a pipeline skewed by one row per level: after row
 // n is loaded, level i (0-based) forms x^(i+1) at row n - i and y^(i+1) at
@@ -68,6 +70,29 @@ __device__ inline const T* byte_off(const T* p, uint32_t bytes) {
   return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (size_t)bytes);
 }
 
+// Strips of an instance W columns wide.  Plainly, strip s holds the exact
+// columns [s sw, (s + 1) sw) (sw = 64 - 2K) on lanes K .. 63 - K, lane 0 on
+// column s sw - K: ceil(W / sw) strips.  Extended, when that takes fewer
+// strips: the first strip needs no halo beyond the left image edge, whose
+// boundary the stencils' masks close, so lane 0 sits on column 0 and lanes
+// 0 .. 63 - K are exact; the last (the one whose lanes reach column W - 1)
+// needs none beyond the right edge, so its exact lanes run to 63; lane 0 of
+// strip s sits on column s sw, and n strips cover 64 + (n - 1) sw columns.
+// The plain layout keeps the strip edges where they are better aligned for
+// the row loads and stores (3-7% faster at equal strip counts), which wins
+// where HBM bounds the launch: the extended one is taken only from K = 6
+// (sw <= 52), where instruction issue does (1024 columns at K = 8: 21
+// strips instead of 22, 3.5% faster; at K = 2: 17 instead of 18, but 3.5%
+// slower).
+__host__ __device__ inline int stream_strips_plain(int W, int sw) { return (W + sw - 1) / sw; }
+__host__ __device__ inline int stream_strips_ext(int W, int sw) { return W <= 64 ? 1 : (W - 64 + sw - 1) / sw + 1; }
+__host__ __device__ inline bool stream_extended(int W, int sw) {
+  return sw <= 52 && stream_strips_ext(W, sw) < stream_strips_plain(W, sw);
+}
+__host__ __device__ inline int stream_strips(int W, int sw) {
+  return stream_extended(W, sw) ? stream_strips_ext(W, sw) : stream_strips_plain(W, sw);
+}
+
 // plain vector stores (the rows of adjacent strips share cache lines, which
 // the L2 merges; measured no slower than streaming stores here)
 template <typename T, int N>
@@ -98,10 +123,18 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (int u = xcd_block() * (kTB / 64) + wave; u < total; u += gridDim.x * (kTB / 64)) {
-    const TileRef tr = locate(u, active_inst, unit_off, nact, insts, SW, R);
+    TileRef tr = locate(u, active_inst, unit_off, nact, insts, SW, R);
+    {
+      const int ns = stream_strips(insts[tr.b].W, SW), ty = tr.j / ns;
+      tr.r0 = ty * R;
+      tr.c0 = (tr.j - ty * ns) * SW;
+    }
     const opy_tv_inst_t I = insts[tr.b];
     const int H = I.H, W = I.W;
-    const int r0 = tr.r0 - K, c0 = tr.c0 - K;  // image row of pipeline row 0, image column of lane 0
+    const bool ext = stream_extended(W, SW);
+    const int r0 = tr.r0 - K, c0 = ext ? tr.c0 : tr.c0 - K;  // image row of pipeline row 0, column of lane 0
+    // exact lanes, stored: [slo, shi)
+    const int slo = (ext && c0 == 0) ? 0 : K, shi = (ext && c0 + 64 >= W) ? 64 : 64 - K;
     const int c = c0 + lane;
     FoldedStep<T, C> fs[K];
     {
@@ -214,7 +247,7 @@ __global__ void __launch_bounds__(kTB) tv_stream_kernel(const T* __restrict__ xi
         // xcur = x^K, ycur = y^K at pipeline row n - K
         const int m = n - K;
         const int r = r0 + m;
-        if (m >= K && r < H && lane >= K && lane < 64 - K && c < W) {
+        if (m >= K && r < H && lane >= slo && lane < shi && c < W) {
           const uint32_t p = (uint32_t)(r * W) + (uint32_t)c;
           store_v<T, C>(byte_off(xob, p * (uint32_t)(C * sizeof(T))), xcur);
           store_v<T, 2 * C>(byte_off(yob, p * (uint32_t)(2 * C * sizeof(T))), ycur);
@@ -271,7 +304,7 @@ __global__ void __launch_bounds__(kTB) tv_stream_units_kernel(const opy_tv_inst_
   int n = 0;
   for (int a = lo; a < hi; ++a) {
     const opy_tv_inst_t& I = insts[active_inst[a]];
-    n += ((I.W + sw - 1) / sw) * ((I.H + rows - 1) / rows);
+    n += stream_strips(I.W, sw) * ((I.H + rows - 1) / rows);
   }
   part[threadIdx.x] = n;
   __syncthreads();
@@ -289,7 +322,7 @@ __global__ void __launch_bounds__(kTB) tv_stream_units_kernel(const opy_tv_inst_
   for (int a = lo; a < hi; ++a) {
     unit_off[a] = t;
     const opy_tv_inst_t& I = insts[active_inst[a]];
-    t += ((I.W + sw - 1) / sw) * ((I.H + rows - 1) / rows);
+    t += stream_strips(I.W, sw) * ((I.H + rows - 1) / rows);
   }
 }
 

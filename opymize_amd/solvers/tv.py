@@ -108,6 +108,17 @@ def exported_names():
     return [k + s for k in _SIGS for s in ("_f32", "_f64")] + list(_PLAIN)
 
 
+def _stream_strips(W, sw):
+    """Strips of the streaming kernel for an instance W columns wide (the
+    kernel's stream_strips): ceil(W / sw) strips of sw = 64 - 2K exact
+    columns, or, from K = 6 on when that takes fewer, the extended layout
+    whose first and last strips need no halo beyond the image edge,
+    64 + (n - 1) sw columns for n strips."""
+    plain = -(-W // sw)
+    ext = 1 if W <= 64 else -(-(W - 64) // sw) + 1
+    return min(plain, ext) if sw <= 52 else plain
+
+
 def tile_height(tw, rpt):
     return (256 // tw) * rpt
 
@@ -222,7 +233,7 @@ class TVBatch(object):
             self.total_units = 0
             return
         sw, rows = 64 - 2 * self.multi, self.stream_rows
-        self.total_units = sum((-(-W // sw)) * (-(-H // rows)) for (H, W) in self.shapes)
+        self.total_units = sum(_stream_strips(W, sw) * (-(-H // rows)) for (H, W) in self.shapes)
         if self._grid_override is None:
             self.sgrid = self._stream_grid(self.total_units)
         else:

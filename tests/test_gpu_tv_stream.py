@@ -81,7 +81,9 @@ def test_stream_row_phases_equal_single_launches(gpu, precision, multi, rows):
     masks: images wide enough for such strips, from 5 rows (no unmasked
     step) to 200 (units away from both edges), give the same bits as
     single-iteration launches."""
-    shapes = [(200, 300), (5, 300), (21, 260), (130, 130), (17, 700)]
+    # widths around one and two strips (64, and 64 + 64 - 2K for K = 6, 8)
+    shapes = [(200, 300), (5, 300), (21, 260), (130, 130), (17, 700), (12, 65), (30, 112), (31, 113), (9, 116),
+              (11, 117)]
     out = {}
     for m in (1, multi):
         bat, *_ = _batch(precision, m, "stream", shapes=shapes, seed=13, rows=rows)
@@ -181,12 +183,14 @@ def test_auto_switches_to_tiles_when_few_units_run(gpu):
         np.testing.assert_array_equal(a, b)
 
 
-def test_stream_units_follow_the_running_instances(gpu):
+@pytest.mark.parametrize("multi", [4, 8])
+def test_stream_units_follow_the_running_instances(gpu, multi):
     """With some instances marked certified, the compacted unit prefix covers
     exactly the running instances (host recount), the next launches are
-    sized to it, and a launch leaves the stopped instances untouched."""
+    sized to it, and a launch leaves the stopped instances untouched (K = 8:
+    the extended strip layout)."""
     import torch
-    bat, imgs, lams, data = _batch("float", 4, "stream", seed=3, data="l2", rows=16)
+    bat, imgs, lams, data = _batch("float", multi, "stream", seed=3, data="l2", rows=16)
     bat.solve(steps="accel", step_bound=0.124, max_iters=8, block_iters=8, tol=1e-30)
     stopped = [1, 4, 5]
     ck = bat.checks.view(torch.int32).reshape(bat.B, -1)
@@ -197,12 +201,16 @@ def test_stream_units_follow_the_running_instances(gpu):
     active = bat.active_inst.cpu().numpy()[:nrun]
     assert nrun == bat.B - len(stopped) and not set(stopped) & set(active.tolist())
     uo = bat.unit_off.cpu().numpy()
-    sw = 64 - 2 * 4
-    counts = [(-(-bat.shapes[b][1] // sw)) * (-(-bat.shapes[b][0] // 16)) for b in active]
+    sw = 64 - 2 * multi
+    # ceil(W / sw) strips, or from K = 6 on the extended layout when it takes
+    # fewer (no halo beyond the image edges: 64 + (n - 1) sw columns)
+    strips = [-(-W // sw) if multi < 6 else min(-(-W // sw), 1 if W <= 64 else 1 + -(-(W - 64) // sw))
+              for (H, W) in bat.shapes]
+    counts = [strips[b] * (-(-bat.shapes[b][0] // 16)) for b in active]
     np.testing.assert_array_equal(uo[:nrun + 1], np.concatenate([[0], np.cumsum(counts)]))
     assert bat.sgrid * 4 >= uo[nrun] and bat.sgrid % 8 == 0
     x0 = bat.x[1 - bat.cur].clone()
-    bat._multi_launch(bat.cur, bat._cfg("accel"), 4)
+    bat._multi_launch(bat.cur, bat._cfg("accel"), multi)
     torch.cuda.synchronize()
     xs = bat.unpack(bat.x[1 - bat.cur], "x")
     xo = bat.unpack(x0, "x")
