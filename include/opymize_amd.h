@@ -426,7 +426,9 @@ int opy_tv_stream_f64(const double* x_in, double* x_out, const double* y_in, dou
                       const int32_t* unit_off, const int32_t* nact, const opy_tv_cfg_t* cfg, int k, int rows,
                       int grid, void* stream);
 /* unit_off[a] = sum over the running instances a' < a (active_inst, nact as from opy_tv_compact) of
- * ceil(W / (64 - 2k)) * ceil(H / rows); unit_off[nact] = the total.  One block. */
+ * strips(W) * ceil(H / rows); unit_off[nact] = the total.  One block.  strips(W) = ceil(W / sw),
+ * sw = 64 - 2k, or for k >= 6, when that is fewer, the strips of the layout whose first and last
+ * strips carry no halo beyond the image edge: 1 for W <= 64, else ceil((W - 64) / sw) + 1. */
 int opy_tv_stream_units(const opy_tv_inst_t* insts, const int32_t* active_inst, const int32_t* nact,
                         int32_t* unit_off, int k, int rows, void* stream);
 /* adaptive step rule per running instance from the residual partials (reference pdhg.py:155-183) */
