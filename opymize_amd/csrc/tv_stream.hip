@@ -58,7 +58,9 @@ __device__ inline FoldedStep<T, C> bcast_step(const FoldedStep<T, C>& s, int l) 
   return o;
 }
 
-constexpr int kStreamD = 2;  // rows loaded ahead of the pipeline
+// rows loaded ahead of the pipeline: 1 (no register ring to shift; 1-3% faster than 2 or 3 at
+// 6 and 8 iterations per launch, equal below, where other wavefronts hide the loads' latency)
+constexpr int kStreamD = 1;
 
 // p + a 32-bit byte offset: a uniform base and a zero-extended vector offset
 template <typename T>
